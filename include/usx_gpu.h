@@ -1,0 +1,191 @@
+/* usx_gpu.h -- C ABI of the MI355X (gfx950) imaging path.
+ *
+ * This is the drop-in boundary below the reference's C++ API. The reference
+ * has no C ABI of its own: its boundary is pybind11 over C++ classes
+ * (bindings/pymodule.cpp). This library is what the C++17 host layer of this
+ * repository (ultrasound-engine_amd/core) calls from the same three C++
+ * entry points the reference exposes, which bindings/pymodule.cpp binds
+ * unchanged:
+ *
+ *   usx_demodulate     replaces  ChannelData demodulate(const ChannelData&, DemodConfig)
+ *                                core/include/usx/demodulate.hpp:45,
+ *                                core/src/demodulate.cpp:36-121
+ *   usx_beamform       replaces  Frame Beamformer::beamform(const ChannelData&,
+ *                                const ScanGrid&, int, int) const
+ *                                core/include/usx/beamformer.hpp:114-115,
+ *                                core/src/beamformer.cpp:297-549
+ *   usx_process_frame  replaces  Frame ImagingPipeline::process_frame(const ChannelData&)
+ *                                core/include/usx/pipeline.hpp:52,
+ *                                core/src/pipeline.cpp:25-46
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; no C++ or torch types cross the ABI.
+ *   - Channel data is [event][channel][sample], sample fastest
+ *     (ChannelData::index, types.hpp:206-209). Frames are [axis0][axis1],
+ *     axis1 fastest (Frame::at, types.hpp:278-279).
+ *   - Complex values are interleaved float pairs (usx_c32), bit-compatible
+ *     with std::complex<float> and numpy complex64.
+ *   - `memory` selects the address space of the data pointers: USX_HOST
+ *     means caller-owned host buffers and a synchronous call (upload, launch,
+ *     download, stream sync); USX_DEVICE means device buffers and an
+ *     asynchronous call enqueued on the context's stream.
+ *   - Descriptor arrays (taps, rotator, element_x, transmits, axes) are
+ *     always host memory, read during the call.
+ *   - No exception crosses the ABI. Every call returns a usx_status; the
+ *     message of the last failure is usx_ctx_last_error(ctx). The C++ layer
+ *     maps USX_ERR_INVALID to std::invalid_argument and the rest to
+ *     std::runtime_error, so Python sees ValueError / RuntimeError as with the
+ *     reference (pymodule.cpp maps them the same way).
+ *   - One context per pipeline or beamformer object. Calls on one context
+ *     must not overlap (the reference's ImagingPipeline is not re-entrant
+ *     either, pipeline.cpp:27, 43-44); separate contexts are independent.
+ */
+#ifndef USX_GPU_H
+#define USX_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  USX_OK = 0,
+  USX_ERR_INVALID = 1,   /* a descriptor or size is invalid */
+  USX_ERR_RUNTIME = 2,   /* HIP runtime failure (message says which call) */
+  USX_ERR_NO_DEVICE = 3  /* no gfx950 device, or the requested ordinal is absent */
+} usx_status;
+
+enum { USX_HOST = 0, USX_DEVICE = 1 };
+
+typedef struct {
+  float re, im;
+} usx_c32;
+
+typedef struct usx_ctx usx_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Create a context on HIP device `device`: one HIP stream (default
+ * priority), pooled device buffers grown on demand, the last error string. */
+usx_status usx_ctx_create(int device, usx_ctx** out);
+usx_status usx_ctx_destroy(usx_ctx* ctx);
+const char* usx_ctx_last_error(const usx_ctx* ctx);
+/* The context's hipStream_t, for callers that enqueue their own work. */
+void* usx_ctx_stream(usx_ctx* ctx);
+usx_status usx_ctx_synchronize(usx_ctx* ctx);
+/* Device ordinal, or -1 for a null context. */
+int usx_ctx_device(const usx_ctx* ctx);
+/* Library build tag (gfx target and compile options), for diagnostics. */
+const char* usx_build_info(void);
+
+/* Timing and profiling (no reference counterpart; the reference times stages
+ * with std::chrono, pipeline.cpp:6-11). Events are recorded on the context
+ * stream, so they bracket exactly the work enqueued between them. */
+usx_status usx_ctx_event_record(usx_ctx* ctx, int slot);   /* slot 0..7 */
+usx_status usx_ctx_event_elapsed(usx_ctx* ctx, int slot_a, int slot_b, float* ms);
+/* When enabled, every kernel launch is bracketed by events; the report
+ * (after a synchronize) is JSON: {"kernel": [launches, total_ms], ...}
+ * accumulated since profiling was enabled. */
+usx_status usx_ctx_profile(usx_ctx* ctx, int enable);
+usx_status usx_ctx_profile_report(usx_ctx* ctx, char* buf, size_t len);
+
+/* ---- demodulation (demodulate.cpp:36-121) ----------------------------- */
+
+typedef struct {
+  int n_traces;            /* n_events * n_channels */
+  int n_in;                /* RF samples per trace */
+  int n_out;               /* (n_in + decimation - 1) / decimation */
+  int decimation;          /* >= 1 */
+  int n_taps;              /* odd, as design_lowpass returns */
+  const float* taps;       /* host, n_taps: design_lowpass(...) */
+  const usx_c32* rotator;  /* host, n_in: exp(-j 2 pi fdem (t0 + n/fs)) as
+                              the reference tabulates it (demodulate.cpp:72-76) */
+} usx_demod_desc;
+
+/* rf: n_traces * n_in floats -> iq: n_traces * n_out complex. */
+usx_status usx_demodulate(usx_ctx* ctx, const usx_demod_desc* d, const float* rf, usx_c32* iq,
+                          int memory);
+
+/* ---- beamforming (beamformer.cpp:297-549) ----------------------------- */
+
+/* One transmit event with everything the per-pixel delay and illumination
+ * laws need that does not depend on the pixel, precomputed on the host with
+ * the reference's own float expressions (geometry.cpp:82-109,
+ * beamformer.cpp:238-288). */
+typedef struct {
+  int type;              /* 0 Focused, 1 PlaneWave, 2 Diverging */
+  float sin_a, cos_a;    /* sin/cos of the steering angle (plane wave) */
+  float tan_a;           /* tan of the steering angle (plane wave illumination) */
+  float origin_x;        /* beam origin on the face (Transmit::origin.x) */
+  float focus[3];        /* focal point / virtual source */
+  float axis_len;        /* |focus - origin| (focused) */
+  float axis_dir[3];     /* (focus - origin) / axis_len (focused) */
+  float origin_focus;    /* |origin - focus| (diverging) */
+  int extent_valid;      /* firing aperture not empty */
+  float x_lo, x_hi;      /* lateral extent of the firing aperture */
+  float aperture;        /* max(x_hi - x_lo, pitch) */
+} usx_tx_desc;
+
+typedef struct {
+  /* channel data */
+  int n_events;            /* events present in the data */
+  int n_channels;
+  int n_samples;
+  int is_iq;               /* 1: data is usx_c32 IQ; 0: data is float RF */
+  float sampling_frequency, t0, speed_of_sound;
+  float demodulation_frequency;  /* > 0 for IQ; ignored for RF */
+  float wavelength;        /* speed_of_sound / center_frequency */
+  float pitch;
+  const float* element_x;  /* host, n_channels (Probe::element_position) */
+  const usx_tx_desc* tx;   /* host, n_events */
+  int event_begin, event_end;    /* already clamped, 0 <= begin <= end <= n_events */
+  /* grid (ScanGrid::cartesian / sector, types.cpp:66-115) */
+  int grid_type;           /* 0 Cartesian, 1 Sector */
+  int n_axis0, n_axis1;
+  const float* axis0;      /* host, n_axis0 */
+  const float* axis1;      /* host, n_axis1 */
+  const float* axis0_sin;  /* host, n_axis0, sector only: sin(theta) */
+  const float* axis0_cos;  /* host, n_axis0, sector only: cos(theta) */
+  float apex_x, apex_z;
+  /* BeamformerConfig (beamformer.hpp:66-106) */
+  float f_number;
+  int rx_window;           /* 0 Rect 1 Hann 2 Hamming 3 Tukey 4 Blackman */
+  float tukey_alpha;
+  int combiner;            /* 0 DAS 1 DMAS 2 MV */
+  int compounding;         /* 0 Coherent 1 Incoherent 2 Select */
+  int transmit_weighting;
+  int coherence_factor;
+  int mv_subarray;
+  float mv_diagonal_loading;
+  int mv_range_average;
+  /* additions with no reference counterpart (default 0 = reference behaviour) */
+  int mv_precision;        /* 0 FP32 covariance, 1 BF16 inputs, 2 FP16 inputs (FP32 accumulate) */
+  int mv_refine;           /* 1: one step of iterative refinement, FP64 residual */
+} usx_bf_desc;
+
+/* data: n_events * n_channels * n_samples (float RF or usx_c32 IQ);
+ * frame: n_axis0 * n_axis1 complex. */
+usx_status usx_beamform(usx_ctx* ctx, const usx_bf_desc* b, const void* data, usx_c32* frame,
+                        int memory);
+
+/* Stage times of the last call, from HIP events on the context stream
+ * (PipelineStats fields, pipeline.hpp:38-44). */
+typedef struct {
+  double demodulate_ms;
+  double beamform_ms;
+  double total_ms;
+} usx_stage_times;
+
+/* RF in, frame out: demodulate (d) then beamform (b, whose n_samples,
+ * sampling_frequency, demodulation_frequency and is_iq describe the IQ that
+ * d produces). times may be NULL. */
+usx_status usx_process_frame(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
+                             const float* rf, usx_c32* frame, int memory,
+                             usx_stage_times* times);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* USX_GPU_H */

@@ -1,0 +1,513 @@
+// beamform.hip -- delay, interpolate, rotate and combine on gfx950.
+//
+// Reference: Beamformer::beamform, core/src/beamformer.cpp:297-549.
+//
+// Thread mapping. One thread owns one pixel. A workgroup is 4 wavefronts;
+// wavefront w of a workgroup covers 64 consecutive axis1 (depth) pixels of
+// lateral line i0 = 4*g + w, so the four waves of a workgroup read the same
+// IQ sample windows for a given (event, channel) and share them in L1. The
+// workgroup index is remapped so that each XCD (8 per MI355X, each with its
+// own L2) receives a contiguous range of tiles.
+//
+// Per thread: the transmit terms of each event slot (delay, illumination
+// weight times transmit phasor) are computed once and held in registers;
+// then the channel loop walks the union of the wavefront's active apertures,
+// computing the receive terms (aperture weight, delay, phasor) once per
+// (pixel, channel) and accumulating over the event slots:
+//
+//   Y_ch = (a_ch e^{j phi_rx}) * sum_e v_e(idx) * (tw_e e^{j phi_tx,e})
+//
+// which is the reference's Y[ch] += v * (e^{j phi_tx} e^{j phi_rx}) * (a*tw)
+// (beamformer.cpp:470-478) with the receive factor taken out of the event
+// sum. The interpolation index, whose rounding decides which samples are
+// read, is evaluated in the reference's exact operation order.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace usxd {
+
+namespace {
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Tile decode with XCD-aware remapping of the workgroup index.
+__device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int per = (nb + 7) / 8;
+  int t = (b % 8) * per + b / 8;
+  // Blocks beyond an uneven last XCD slice fall back to the identity map.
+  if (nb % 8 != 0) t = b;
+  const int nj = (A.g.n1 + 63) / 64;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int tg = t / nj, tj = t - tg * nj;
+  *i0 = tg * 4 + wave;
+  *j = tj * 64 + lane;
+}
+
+struct TxSlot {
+  float tau, tr, ti;
+  int ev;
+};
+
+// Transmit terms of one event for pixel p (beamformer.cpp:425-451).
+__device__ __forceinline__ TxSlot tx_slot(const BfArgs& A, int ev, float w, const Pt& p) {
+  TxSlot s;
+  s.ev = ev;
+  s.tau = tx_time(A.tx[ev], p, A.c);
+  if (A.fdem > 0.0f) {
+    float sn, cs;
+    sincosf(A.w0 * s.tau, &sn, &cs);
+    s.tr = w * cs;
+    s.ti = w * sn;
+  } else {
+    s.tr = w;
+    s.ti = 0.0f;
+  }
+  return s;
+}
+
+// Illumination weight of event ev for pixel p, or of the Select winner.
+__device__ __forceinline__ float tx_w(const BfArgs& A, int ev, const Pt& p) {
+  return A.transmit_weighting ? tx_weight(A.tx[ev], p, A.lambda) : 1.0f;
+}
+
+// Best-illuminating event (Select compounding, beamformer.cpp:394-408):
+// first strict maximum of the illumination weight, -1 if none is lit.
+__device__ __forceinline__ int select_event(const BfArgs& A, const Pt& p, float* wbest) {
+  int best = -1;
+  float bw = 0.0f;
+  for (int e = A.ev_begin; e < A.ev_end; ++e) {
+    const float w = tx_weight(A.tx[e], p, A.lambda);
+    if (w > bw) {
+      bw = w;
+      best = e;
+    }
+  }
+  *wbest = bw;
+  return best;
+}
+
+// Linear interpolation of a trace at a fractional index (beamformer.cpp:27-39);
+// zero when floor(idx) < 0 or floor(idx) + 1 >= n (and for NaN).
+template <bool IQ>
+__device__ __forceinline__ float2 sample(const void* data, size_t base, int n, float idx) {
+  if (!(idx >= 0.0f && idx < static_cast<float>(n - 1))) return make_float2(0.0f, 0.0f);
+  const int k = static_cast<int>(floorf(idx));
+  const float f = idx - static_cast<float>(k);
+  const float g = 1.0f - f;
+  if (IQ) {
+    const float2* t = static_cast<const float2*>(data) + base;
+    const float2 a = t[k], b = t[k + 1];
+    return make_float2(a.x * g + b.x * f, a.y * g + b.y * f);
+  }
+  const float* t = static_cast<const float*>(data) + base;
+  return make_float2(t[k] * g + t[k + 1] * f, 0.0f);
+}
+
+// Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
+__device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, bool valid, int* lo,
+                                               int* hi) {
+  int l = INT_MAX, h = -1;
+  if (valid && p.z > 0.0f) {
+    const float half = 0.5f * p.z / smax(A.f_number, 1e-3f);
+    const float c0 = 0.5f * static_cast<float>(A.n_ch - 1);
+    const float fl = (p.x - half) / A.pitch + c0;
+    const float fh = (p.x + half) / A.pitch + c0;
+    l = fl < -1.0f ? 0 : static_cast<int>(floorf(fl)) - 1;
+    h = fh > static_cast<float>(A.n_ch) ? A.n_ch - 1 : static_cast<int>(ceilf(fh)) + 1;
+    l = max(l, 0);
+    h = min(h, A.n_ch - 1);
+    if (!(half == half)) {  // NaN geometry: scan everything, exact tests decide
+      l = 0;
+      h = A.n_ch - 1;
+    }
+  }
+  *lo = wave_min(l);
+  *hi = wave_max(h);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fused kernel: channel loop and DAS / DMAS combine (+ coherence factor) in
+// registers. NE = event slots per launch (compile-time, <= 16).
+//   select   : one slot per pixel, the pixel's best event, weight 1
+//   accumulate: add to the existing frame (DAS over event chunks; linear)
+// ---------------------------------------------------------------------------
+template <int NE, int COMB, bool CF, bool IQ>
+__global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* data, float2* frame,
+                                                       int ev_first, int n_slots, int select,
+                                                       int accumulate) {
+  int i0, j;
+  tile_of(A, &i0, &j);
+  const bool valid = i0 < A.g.n0 && j < A.g.n1;
+  const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
+
+  TxSlot sl[NE];
+  int nsl = n_slots;
+  if (select) {
+    float wb;
+    const int e = valid ? select_event(A, p, &wb) : -1;
+    sl[0] = tx_slot(A, e < 0 ? A.ev_begin : e, (e >= 0 && wb > 0.0f) ? 1.0f : 0.0f, p);
+    nsl = 1;
+  } else {
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n_slots) sl[s] = tx_slot(A, ev_first + s, valid ? tx_w(A, ev_first + s, p) : 0.0f, p);
+  }
+
+  int lo, hi;
+  aperture_range(A, p, valid, &lo, &hi);
+
+  float2 das = make_float2(0.0f, 0.0f);
+  float2 ds = das, dq = das, coh = das;
+  float inc = 0.0f;
+  int na = 0;
+  for (int ch = lo; ch <= hi; ++ch) {
+    const float ex = A.ex[ch];
+    const float a = valid ? aperture_weight(ex, p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
+    if (a <= 0.0f) continue;
+    ++na;
+    const float trx = rx_time(ex, p, A.c);
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      if (s < nsl) {
+        const float idx = (sl[s].tau + trx - A.t0) * A.fs;
+        const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
+        const float2 v = sample<IQ>(data, base, A.n_s, idx);
+        acc.x += v.x * sl[s].tr - v.y * sl[s].ti;
+        acc.y += v.x * sl[s].ti + v.y * sl[s].tr;
+      }
+    }
+    float rr = a, ri = 0.0f;
+    if (A.fdem > 0.0f) {
+      float sn, cs;
+      sincosf(A.w0 * trx, &sn, &cs);
+      rr = a * cs;
+      ri = a * sn;
+    }
+    const float2 y = make_float2(acc.x * rr - acc.y * ri, acc.x * ri + acc.y * rr);
+    if (COMB == 0) {
+      das.x += y.x;
+      das.y += y.y;
+    } else {
+      const float m = hypotf(y.x, y.y);
+      if (m > 0.0f) {
+        const float sq = sqrtf(m);
+        const float vr = y.x / sq, vi = y.y / sq;
+        ds.x += vr;
+        ds.y += vi;
+        dq.x += vr * vr - vi * vi;
+        dq.y += vr * vi + vi * vr;
+      }
+    }
+    if (CF) {
+      coh.x += y.x;
+      coh.y += y.y;
+      const float m = hypotf(y.x, y.y);
+      inc += m * m;
+    }
+  }
+  if (!valid) return;
+  float2 out = make_float2(0.0f, 0.0f);
+  if (na > 0) {
+    if (COMB == 0) {
+      out = das;
+    } else {
+      // ((s*s - q) * 0.5) / (0.5 * max(1, n - 1))  (beamformer.cpp:125-129)
+      const float ar = (ds.x * ds.x - ds.y * ds.y - dq.x) * 0.5f;
+      const float ai = (ds.x * ds.y + ds.y * ds.x - dq.y) * 0.5f;
+      const float nrm = 0.5f * smax(1.0f, static_cast<float>(na - 1));
+      out = make_float2(ar / nrm, ai / nrm);
+    }
+    if (CF) {
+      const float mc = hypotf(coh.x, coh.y);
+      const float f = (inc > 0.0f) ? (mc * mc) / (static_cast<float>(na) * inc) : 0.0f;
+      out.x *= f;
+      out.y *= f;
+    }
+  }
+  float2* o = frame + static_cast<size_t>(i0) * A.g.n1 + j;
+  if (accumulate) {
+    const float2 prev = *o;
+    out.x += prev.x;
+    out.y += prev.y;
+  }
+  *o = out;
+}
+
+// ---------------------------------------------------------------------------
+// Channel-vector kernel: Y[pixel][ch] over event chunks, plus the active mask
+// (one bit per channel, a > 0). rect: MV's rectangular aperture
+// (beamformer.cpp:385). Y is accumulated across chunk launches; the first
+// chunk writes. Inactive channels are written as zero.
+// ---------------------------------------------------------------------------
+template <int NE, bool IQ>
+__global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* data, float2* Y,
+                                                          uint32_t* mask, int ev_first,
+                                                          int n_slots, int select, int first,
+                                                          int rect) {
+  int i0, j;
+  tile_of(A, &i0, &j);
+  const bool valid = i0 < A.g.n0 && j < A.g.n1;
+  const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
+  const size_t pix = static_cast<size_t>(i0) * A.g.n1 + j;
+
+  TxSlot sl[NE];
+  int nsl = n_slots;
+  if (select) {
+    float wb;
+    const int e = valid ? select_event(A, p, &wb) : -1;
+    sl[0] = tx_slot(A, e < 0 ? A.ev_begin : e, (e >= 0 && wb > 0.0f) ? 1.0f : 0.0f, p);
+    nsl = 1;
+  } else {
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n_slots) sl[s] = tx_slot(A, ev_first + s, valid ? tx_w(A, ev_first + s, p) : 0.0f, p);
+  }
+
+  const int nw = (A.n_ch + 31) / 32;
+  if (!valid) return;
+  float2* yp = Y + pix * A.n_ch;
+  uint32_t* mp = mask + pix * nw;
+  uint32_t bits = 0;
+  for (int ch = 0; ch < A.n_ch; ++ch) {
+    const float ex = A.ex[ch];
+    float a = aperture_weight(ex, p, A.f_number, A.rx_window, A.tukey_alpha);
+    if (rect) a = a > 0.0f ? 1.0f : 0.0f;
+    float2 y = make_float2(0.0f, 0.0f);
+    if (a > 0.0f) {
+      bits |= 1u << (ch & 31);
+      const float trx = rx_time(ex, p, A.c);
+      float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int s = 0; s < NE; ++s) {
+        if (s < nsl) {
+          const float idx = (sl[s].tau + trx - A.t0) * A.fs;
+          const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
+          const float2 v = sample<IQ>(data, base, A.n_s, idx);
+          acc.x += v.x * sl[s].tr - v.y * sl[s].ti;
+          acc.y += v.x * sl[s].ti + v.y * sl[s].tr;
+        }
+      }
+      float rr = a, ri = 0.0f;
+      if (A.fdem > 0.0f) {
+        float sn, cs;
+        sincosf(A.w0 * trx, &sn, &cs);
+        rr = a * cs;
+        ri = a * sn;
+      }
+      y = make_float2(acc.x * rr - acc.y * ri, acc.x * ri + acc.y * rr);
+    }
+    if (!first) {
+      const float2 prev = yp[ch];
+      y.x += prev.x;
+      y.y += prev.y;
+    }
+    yp[ch] = y;
+    if ((ch & 31) == 31 || ch == A.n_ch - 1) {
+      mp[ch >> 5] = bits;
+      bits = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Combine kernel from channel vectors: DAS / DMAS (+ CF). incoherent: add
+// |out|^2 into acc instead of writing the frame (beamformer.cpp:536).
+// ---------------------------------------------------------------------------
+template <int COMB, bool CF>
+__global__ __launch_bounds__(256) void bf_combine_kernel(int n_pix, int n_ch, const float2* Y,
+                                                         const uint32_t* mask, float2* frame,
+                                                         float* acc_out, int incoherent) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= n_pix) return;
+  const int nw = (n_ch + 31) / 32;
+  const float2* yp = Y + static_cast<size_t>(pix) * n_ch;
+  const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
+  float2 das = make_float2(0.0f, 0.0f), ds = das, dq = das, coh = das;
+  float inc = 0.0f;
+  int na = 0;
+  for (int w = 0; w < nw; ++w) {
+    uint32_t bits = mp[w];
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      const float2 y = yp[w * 32 + b];
+      ++na;
+      if (COMB == 0) {
+        das.x += y.x;
+        das.y += y.y;
+      } else {
+        const float m = hypotf(y.x, y.y);
+        if (m > 0.0f) {
+          const float sq = sqrtf(m);
+          const float vr = y.x / sq, vi = y.y / sq;
+          ds.x += vr;
+          ds.y += vi;
+          dq.x += vr * vr - vi * vi;
+          dq.y += vr * vi + vi * vr;
+        }
+      }
+      if (CF) {
+        coh.x += y.x;
+        coh.y += y.y;
+        const float m = hypotf(y.x, y.y);
+        inc += m * m;
+      }
+    }
+  }
+  if (na == 0) {
+    if (!incoherent) frame[pix] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  float2 out;
+  if (COMB == 0) {
+    out = das;
+  } else {
+    const float ar = (ds.x * ds.x - ds.y * ds.y - dq.x) * 0.5f;
+    const float ai = (ds.x * ds.y + ds.y * ds.x - dq.y) * 0.5f;
+    const float nrm = 0.5f * smax(1.0f, static_cast<float>(na - 1));
+    out = make_float2(ar / nrm, ai / nrm);
+  }
+  if (CF) {
+    const float mc = hypotf(coh.x, coh.y);
+    const float f = (inc > 0.0f) ? (mc * mc) / (static_cast<float>(na) * inc) : 0.0f;
+    out.x *= f;
+    out.y *= f;
+  }
+  if (incoherent) {
+    const float m = hypotf(out.x, out.y);
+    acc_out[pix] += m * m;
+  } else {
+    frame[pix] = out;
+  }
+}
+
+// Incoherent compounding: frame = (sqrt(acc), 0) (beamformer.cpp:541-543).
+__global__ void incoherent_finish_kernel(int n_pix, const float* acc, float2* frame) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix < n_pix) frame[pix] = make_float2(sqrtf(acc[pix]), 0.0f);
+}
+
+__global__ void fill_zero_kernel(float* p, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static int tile_blocks(const BfArgs& A) {
+  return ((A.g.n0 + 3) / 4) * ((A.g.n1 + 63) / 64);
+}
+
+template <int NE, bool IQ>
+static void launch_fused_ne(const BfArgs& A, const void* data, float2* frame, int ev_first,
+                            int n_slots, int select, int accumulate, int comb, bool cf,
+                            hipStream_t st) {
+  const dim3 grid(tile_blocks(A)), block(256);
+#define USX_FUSED(C_, CF_) \
+  hipLaunchKernelGGL((bf_fused_kernel<NE, C_, CF_, IQ>), grid, block, 0, st, A, data, frame, \
+                     ev_first, n_slots, select, accumulate)
+  if (comb == 0 && !cf) USX_FUSED(0, false);
+  else if (comb == 0 && cf) USX_FUSED(0, true);
+  else if (!cf) USX_FUSED(1, false);
+  else USX_FUSED(1, true);
+#undef USX_FUSED
+}
+
+template <bool IQ>
+static void launch_fused_iq(const BfArgs& A, const void* data, float2* frame, int ev_first,
+                            int n_slots, int select, int accumulate, int comb, bool cf,
+                            hipStream_t st) {
+  if (n_slots <= 1)
+    launch_fused_ne<1, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else if (n_slots <= 4)
+    launch_fused_ne<4, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else if (n_slots <= 8)
+    launch_fused_ne<8, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else
+    launch_fused_ne<16, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+}
+
+void launch_fused(const BfArgs& A, const void* data, float2* frame, int ev_first, int n_slots,
+                  int select, int accumulate, int comb, bool cf, bool iq, hipStream_t st) {
+  if (iq)
+    launch_fused_iq<true>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else
+    launch_fused_iq<false>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+}
+
+template <int NE, bool IQ>
+static void launch_channels_ne(const BfArgs& A, const void* data, float2* Y, uint32_t* mask,
+                               int ev_first, int n_slots, int select, int first, int rect,
+                               hipStream_t st) {
+  hipLaunchKernelGGL((bf_channels_kernel<NE, IQ>), dim3(tile_blocks(A)), dim3(256), 0, st, A,
+                     data, Y, mask, ev_first, n_slots, select, first, rect);
+}
+
+template <bool IQ>
+static void launch_channels_iq(const BfArgs& A, const void* data, float2* Y, uint32_t* mask,
+                               int ev_first, int n_slots, int select, int first, int rect,
+                               hipStream_t st) {
+  if (n_slots <= 1)
+    launch_channels_ne<1, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else if (n_slots <= 4)
+    launch_channels_ne<4, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else if (n_slots <= 8)
+    launch_channels_ne<8, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else
+    launch_channels_ne<16, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+}
+
+void launch_channels(const BfArgs& A, const void* data, float2* Y, uint32_t* mask, int ev_first,
+                     int n_slots, int select, int first, int rect, bool iq, hipStream_t st) {
+  if (iq)
+    launch_channels_iq<true>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else
+    launch_channels_iq<false>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+}
+
+void launch_combine(int n_pix, int n_ch, const float2* Y, const uint32_t* mask, float2* frame,
+                    float* acc, int incoherent, int comb, bool cf, hipStream_t st) {
+  const dim3 grid((n_pix + 255) / 256), block(256);
+#define USX_COMB(C_, CF_) \
+  hipLaunchKernelGGL((bf_combine_kernel<C_, CF_>), grid, block, 0, st, n_pix, n_ch, Y, mask, \
+                     frame, acc, incoherent)
+  if (comb == 0 && !cf) USX_COMB(0, false);
+  else if (comb == 0 && cf) USX_COMB(0, true);
+  else if (!cf) USX_COMB(1, false);
+  else USX_COMB(1, true);
+#undef USX_COMB
+}
+
+void launch_incoherent_finish(int n_pix, const float* acc, float2* frame, hipStream_t st) {
+  hipLaunchKernelGGL(incoherent_finish_kernel, dim3((n_pix + 255) / 256), dim3(256), 0, st,
+                     n_pix, acc, frame);
+}
+
+void launch_fill_zero(float* p, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256),
+                     0, st, p, n);
+}
+
+}  // namespace usxd

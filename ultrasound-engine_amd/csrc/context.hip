@@ -1,0 +1,581 @@
+// context.hip -- the C ABI (include/usx_gpu.h): contexts, pooled device
+// buffers, descriptor validation, launch plans and stage timing.
+//
+// Launch plan of usx_beamform (reference: beamformer.cpp:297-549):
+//   DAS, coherent            fused kernel, events in chunks of <= 16 slots,
+//                            later chunks accumulate (DAS is linear)
+//   DAS/DMAS, Select         fused kernel, one slot (each pixel's best event)
+//   DMAS, coherent <= 16 ev  fused kernel
+//   DMAS, coherent > 16 ev   channel vectors accumulated over chunks, then
+//                            the combine kernel
+//   any, Incoherent          per event: channel vectors, then combine or MV
+//                            with |out|^2 accumulation; sqrt at the end
+//   MV                       channel vectors with the rectangular aperture,
+//                            then the MV kernel (mv.hip)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "usx_gpu.h"
+
+using usxd::BfArgs;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+// Host-side copy of the last uploaded descriptor table, so unchanged tables
+// (the steady state of a pipeline) are not re-uploaded every frame.
+struct Cached {
+  DevBuf dev;
+  std::vector<unsigned char> host;
+};
+
+}  // namespace
+
+struct usx_ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t user_ev[8] = {};
+  std::string err;
+  DevBuf in, out, iq, Y, mask, acc;
+  Cached taps, rot, ex, tx, ax0, ax1, s0, c0;
+  // profiling: pending (name, start, stop) event triples and the totals
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::map<std::string, std::pair<long, double>> totals;
+};
+
+namespace {
+// Set once the process starts exiting; later destroys skip HIP calls, which
+// may run after the runtime has been torn down.
+bool g_exiting = false;
+void mark_exiting() { g_exiting = true; }
+}  // namespace
+
+namespace {
+
+struct Fail {
+  usx_status st;
+};
+
+[[noreturn]] void fail(usx_ctx* c, usx_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  throw Fail{st};
+}
+
+void hip_check(usx_ctx* c, hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(c, USX_ERR_RUNTIME, "%s: %s", what, hipGetErrorString(e));
+}
+
+void* ensure(usx_ctx* c, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.n < bytes) {
+    if (b.p) hip_check(c, hipFree(b.p), "hipFree");
+    b.p = nullptr;
+    b.n = 0;
+    hip_check(c, hipMalloc(&b.p, bytes), "hipMalloc");
+    b.n = bytes;
+  }
+  return b.p;
+}
+
+void* upload(usx_ctx* c, Cached& t, const void* src, size_t bytes) {
+  if (t.dev.n < (bytes == 0 ? 16 : bytes)) t.host.clear();  // reallocation loses the contents
+  void* d = ensure(c, t.dev, bytes);
+  if (bytes == 0) return d;
+  if (t.host.size() != bytes || std::memcmp(t.host.data(), src, bytes) != 0) {
+    t.host.assign(static_cast<const unsigned char*>(src),
+                  static_cast<const unsigned char*>(src) + bytes);
+    hip_check(c, hipMemcpyAsync(d, t.host.data(), bytes, hipMemcpyHostToDevice, c->stream),
+              "hipMemcpyAsync(descriptor)");
+  }
+  return d;
+}
+
+template <typename F>
+usx_status guarded(usx_ctx* c, F&& f) {
+  if (!c) return USX_ERR_INVALID;
+  try {
+    hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+    f();
+    return USX_OK;
+  } catch (const Fail& e) {
+    return e.st;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return USX_ERR_RUNTIME;
+  }
+}
+
+void check_demod(usx_ctx* c, const usx_demod_desc* d) {
+  if (!d) fail(c, USX_ERR_INVALID, "usx_demodulate: null descriptor");
+  if (d->n_traces <= 0 || d->n_in <= 0) fail(c, USX_ERR_INVALID, "demodulate: input has no RF samples");
+  if (d->decimation < 1) fail(c, USX_ERR_INVALID, "demodulate: decimation must be >= 1");
+  if (d->n_out != (d->n_in + d->decimation - 1) / d->decimation)
+    fail(c, USX_ERR_INVALID, "demodulate: n_out must be ceil(n_in / decimation)");
+  if (d->n_taps < 1 || !d->taps || !d->rotator)
+    fail(c, USX_ERR_INVALID, "demodulate: taps and rotator tables are required");
+}
+
+void check_bf(usx_ctx* c, const usx_bf_desc* b) {
+  if (!b) fail(c, USX_ERR_INVALID, "usx_beamform: null descriptor");
+  if (b->n_events <= 0) fail(c, USX_ERR_INVALID, "ChannelData has no transmit events");
+  if (b->n_channels <= 0) fail(c, USX_ERR_INVALID, "Probe.n_elements must be > 0");
+  if (b->n_samples <= 0) fail(c, USX_ERR_INVALID, "Acquisition.n_samples must be > 0");
+  if (b->event_begin < 0 || b->event_end < b->event_begin || b->event_end > b->n_events)
+    fail(c, USX_ERR_INVALID, "beamform: event range [%d, %d) outside [0, %d]", b->event_begin,
+         b->event_end, b->n_events);
+  if (b->n_axis0 <= 0 || b->n_axis1 <= 0) fail(c, USX_ERR_INVALID, "ScanGrid dimensions must be > 0");
+  if (!b->element_x || !b->tx || !b->axis0 || !b->axis1)
+    fail(c, USX_ERR_INVALID, "beamform: element, transmit and grid tables are required");
+  if (b->grid_type == 1 && (!b->axis0_sin || !b->axis0_cos))
+    fail(c, USX_ERR_INVALID, "beamform: sector grids need the sin/cos tables");
+  if (b->grid_type != 0 && b->grid_type != 1) fail(c, USX_ERR_INVALID, "beamform: unknown grid type");
+  if (!(b->f_number > 0)) fail(c, USX_ERR_INVALID, "BeamformerConfig.f_number must be > 0");
+  if (b->mv_subarray < 2) fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_subarray must be >= 2");
+  if (b->mv_diagonal_loading < 0)
+    fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_diagonal_loading must be >= 0");
+  if (b->mv_range_average < 0)
+    fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_range_average must be >= 0");
+  if (b->rx_window < 0 || b->rx_window > 4) fail(c, USX_ERR_INVALID, "unknown window");
+  if (b->combiner < 0 || b->combiner > 2) fail(c, USX_ERR_INVALID, "unknown combiner");
+  if (b->compounding < 0 || b->compounding > 2) fail(c, USX_ERR_INVALID, "unknown compounding");
+  if (b->is_iq && !(b->demodulation_frequency > 0))
+    fail(c, USX_ERR_INVALID, "IQ ChannelData needs a positive demodulation_frequency");
+  if (b->mv_precision < 0 || b->mv_precision > 2) fail(c, USX_ERR_INVALID, "unknown mv_precision");
+  if (b->combiner == 2) {
+    const int lmax = std::min(std::max(b->mv_subarray, 2), std::max(2, b->n_channels / 2));
+    const int pmax = (lmax + 15) & ~15;
+    const size_t lds = static_cast<size_t>(b->n_channels) * 4 + 16 +
+                       8 * (static_cast<size_t>(2 * b->mv_range_average + 1) * b->n_channels +
+                            static_cast<size_t>(pmax) * pmax + 3 * pmax);
+    if (lds > 65536)
+      fail(c, USX_ERR_INVALID,
+           "MV: subarray %d with %d channels and range average %d needs %zu B of LDS per pixel "
+           "(limit 65536)", lmax, b->n_channels, b->mv_range_average, lds);
+  }
+}
+
+hipEvent_t pool_event(usx_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hip_check(c, hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+// Enqueue `fn` and, when profiling, bracket it with events named `name`.
+template <typename F>
+void timed(usx_ctx* c, const char* name, F&& fn) {
+  if (!c->profiling) {
+    fn();
+    return;
+  }
+  hipEvent_t a = pool_event(c), b = pool_event(c);
+  hip_check(c, hipEventRecord(a, c->stream), "hipEventRecord");
+  fn();
+  hip_check(c, hipEventRecord(b, c->stream), "hipEventRecord");
+  c->pending.push_back({name, {a, b}});
+}
+
+void drain_profile(usx_ctx* c) {
+  for (auto& p : c->pending) {
+    float ms = 0;
+    hip_check(c, hipEventSynchronize(p.second.second), "hipEventSynchronize");
+    hip_check(c, hipEventElapsedTime(&ms, p.second.first, p.second.second), "hipEventElapsedTime");
+    auto& t = c->totals[p.first];
+    t.first += 1;
+    t.second += ms;
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->pending.clear();
+}
+
+BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
+  BfArgs A{};
+  A.n_ch = b->n_channels;
+  A.n_s = b->n_samples;
+  A.n_ev = b->n_events;
+  A.ev_begin = b->event_begin;
+  A.ev_end = b->event_end;
+  A.fs = b->sampling_frequency;
+  A.t0 = b->t0;
+  A.c = b->speed_of_sound;
+  A.fdem = b->is_iq ? b->demodulation_frequency : 0.0f;
+  // (2.0f * kPi) * fdem, the reference's left-to-right product (beamformer.cpp:390, 446).
+  A.w0 = (2.0f * usxd::kPi) * A.fdem;
+  A.lambda = b->wavelength;
+  A.pitch = b->pitch;
+  A.f_number = b->f_number;
+  A.rx_window = b->rx_window;
+  A.tukey_alpha = b->tukey_alpha;
+  A.transmit_weighting = b->transmit_weighting;
+  A.ex = static_cast<const float*>(upload(c, c->ex, b->element_x, sizeof(float) * b->n_channels));
+  A.tx = static_cast<const usx_tx_desc*>(
+      upload(c, c->tx, b->tx, sizeof(usx_tx_desc) * b->n_events));
+  A.g.type = b->grid_type;
+  A.g.n0 = b->n_axis0;
+  A.g.n1 = b->n_axis1;
+  A.g.axis0 = static_cast<const float*>(upload(c, c->ax0, b->axis0, sizeof(float) * b->n_axis0));
+  A.g.axis1 = static_cast<const float*>(upload(c, c->ax1, b->axis1, sizeof(float) * b->n_axis1));
+  if (b->grid_type == 1) {
+    A.g.s0 = static_cast<const float*>(upload(c, c->s0, b->axis0_sin, sizeof(float) * b->n_axis0));
+    A.g.c0 = static_cast<const float*>(upload(c, c->c0, b->axis0_cos, sizeof(float) * b->n_axis0));
+  } else {
+    A.g.s0 = A.g.axis0;
+    A.g.c0 = A.g.axis0;
+  }
+  A.g.apex_x = b->apex_x;
+  A.g.apex_z = b->apex_z;
+  return A;
+}
+
+constexpr int kSlots = 16;
+
+// Enqueue the beamforming of device-resident data into a device frame.
+void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* d_frame) {
+  const BfArgs A = make_args(c, b);
+  const bool iq = b->is_iq != 0;
+  const int n_pix = b->n_axis0 * b->n_axis1;
+  const int n_ev = b->event_end - b->event_begin;
+  const bool cf = b->coherence_factor != 0;
+  const int comb = b->combiner;
+  const bool select = b->compounding == 2;
+  const bool incoh = b->compounding == 1;
+  hipStream_t st = c->stream;
+
+  if (n_ev == 0) {  // nothing to beamform: the reference leaves the frame zero
+    timed(c, "fill_zero", [&] { usxd::launch_fill_zero(reinterpret_cast<float*>(d_frame), 2 * static_cast<size_t>(n_pix), st); });
+    return;
+  }
+  const size_t ny = static_cast<size_t>(n_pix) * b->n_channels;
+  const size_t nmask = static_cast<size_t>(n_pix) * ((b->n_channels + 31) / 32);
+  auto Ybuf = [&]() { return static_cast<float2*>(ensure(c, c->Y, ny * sizeof(float2))); };
+  auto Mbuf = [&]() { return static_cast<uint32_t*>(ensure(c, c->mask, nmask * sizeof(uint32_t))); };
+
+  usxd::MvArgs M{};
+  M.n0 = b->n_axis0;
+  M.n1 = b->n_axis1;
+  M.n_ch = b->n_channels;
+  M.subarray = b->mv_subarray;
+  M.loading = b->mv_diagonal_loading;
+  M.range_avg = b->mv_range_average;
+  M.cf = cf;
+  M.precision = b->mv_precision;
+  M.refine = b->mv_refine;
+
+  if (incoh) {
+    float* acc = static_cast<float*>(ensure(c, c->acc, n_pix * sizeof(float)));
+    timed(c, "fill_zero", [&] { usxd::launch_fill_zero(acc, n_pix, st); });
+    float2* Y = Ybuf();
+    uint32_t* mk = Mbuf();
+    for (int e = b->event_begin; e < b->event_end; ++e) {
+      timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e, 1, 0, 1, comb == 2, iq, st); });
+      if (comb == 2) {
+        M.incoherent = 1;
+        timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, st); });
+      } else {
+        timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, acc, 1, comb, cf, st); });
+      }
+    }
+    timed(c, "incoherent_finish", [&] { usxd::launch_incoherent_finish(n_pix, acc, d_frame, st); });
+    return;
+  }
+
+  if (comb == 2) {
+    float2* Y = Ybuf();
+    uint32_t* mk = Mbuf();
+    if (select) {
+      timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, b->event_begin, 1, 1, 1, 1, iq, st); });
+    } else {
+      for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
+        const int n = std::min(kSlots, b->event_end - e0);
+        timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e0, n, 0, e0 == b->event_begin, 1, iq, st); });
+      }
+    }
+    M.incoherent = 0;
+    timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, nullptr, st); });
+    return;
+  }
+
+  if (select) {
+    timed(c, "bf_fused", [&] { usxd::launch_fused(A, d_data, d_frame, b->event_begin, 1, 1, 0, comb, cf, iq, st); });
+    return;
+  }
+  if (comb == 0 && !cf) {
+    for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
+      const int n = std::min(kSlots, b->event_end - e0);
+      timed(c, "bf_fused", [&] { usxd::launch_fused(A, d_data, d_frame, e0, n, 0, e0 != b->event_begin, 0, false, iq, st); });
+    }
+    return;
+  }
+  if (n_ev <= kSlots) {
+    timed(c, "bf_fused", [&] { usxd::launch_fused(A, d_data, d_frame, b->event_begin, n_ev, 0, 0, comb, cf, iq, st); });
+    return;
+  }
+  float2* Y = Ybuf();
+  uint32_t* mk = Mbuf();
+  for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
+    const int n = std::min(kSlots, b->event_end - e0);
+    timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e0, n, 0, e0 == b->event_begin, 0, iq, st); });
+  }
+  timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, nullptr, 0, comb, cf, st); });
+}
+
+void run_demod(usx_ctx* c, const usx_demod_desc* d, const float* d_rf, float2* d_iq) {
+  const float* taps = static_cast<const float*>(upload(c, c->taps, d->taps, sizeof(float) * d->n_taps));
+  const float2* rot =
+      static_cast<const float2*>(upload(c, c->rot, d->rotator, sizeof(usx_c32) * d->n_in));
+  timed(c, "demod", [&] {
+    usxd::launch_demod(d->n_traces, d->n_in, d->n_out, d->decimation, d->n_taps, taps, rot, d_rf,
+                       d_iq, c->stream);
+  });
+}
+
+void check_launch(usx_ctx* c, const char* what) { hip_check(c, hipGetLastError(), what); }
+
+
+
+size_t data_bytes(const usx_bf_desc* b) {
+  return static_cast<size_t>(b->n_events) * b->n_channels * b->n_samples *
+         (b->is_iq ? sizeof(usx_c32) : sizeof(float));
+}
+
+}  // namespace
+
+extern "C" {
+
+usx_status usx_ctx_create(int device, usx_ctx** out) {
+  if (!out) return USX_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return USX_ERR_NO_DEVICE;
+  if (device < 0) device = 0;
+  if (device >= n) return USX_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return USX_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return USX_ERR_NO_DEVICE;
+  static bool registered = (std::atexit(mark_exiting), true);
+  (void)registered;
+  usx_ctx* c = new usx_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return USX_ERR_RUNTIME;
+  }
+  for (auto& e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      usx_ctx_destroy(c);
+      return USX_ERR_RUNTIME;
+    }
+  }
+  *out = c;
+  return USX_OK;
+}
+
+usx_status usx_ctx_destroy(usx_ctx* c) {
+  if (!c) return USX_OK;
+  if (g_exiting) {  // the runtime may already be gone; the OS reclaims everything
+    delete c;
+    return USX_OK;
+  }
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->in, &c->out, &c->iq, &c->Y, &c->mask, &c->acc})
+    if (b->p) (void)hipFree(b->p);
+  for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0})
+    if (t->dev.p) (void)hipFree(t->dev.p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->user_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return USX_OK;
+}
+
+const char* usx_ctx_last_error(const usx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* usx_ctx_stream(usx_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int usx_ctx_device(const usx_ctx* c) { return c ? c->device : -1; }
+
+usx_status usx_ctx_synchronize(usx_ctx* c) {
+  return guarded(c, [&] { hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"); });
+}
+
+usx_status usx_ctx_event_record(usx_ctx* c, int slot) {
+  return guarded(c, [&] {
+    if (slot < 0 || slot >= 8) fail(c, USX_ERR_INVALID, "event slot %d outside 0..7", slot);
+    if (!c->user_ev[slot]) hip_check(c, hipEventCreate(&c->user_ev[slot]), "hipEventCreate");
+    hip_check(c, hipEventRecord(c->user_ev[slot], c->stream), "hipEventRecord");
+  });
+}
+
+usx_status usx_ctx_event_elapsed(usx_ctx* c, int a, int b, float* ms) {
+  return guarded(c, [&] {
+    if (a < 0 || a >= 8 || b < 0 || b >= 8 || !c->user_ev[a] || !c->user_ev[b] || !ms)
+      fail(c, USX_ERR_INVALID, "event slots %d/%d not recorded", a, b);
+    hip_check(c, hipEventSynchronize(c->user_ev[b]), "hipEventSynchronize");
+    hip_check(c, hipEventElapsedTime(ms, c->user_ev[a], c->user_ev[b]), "hipEventElapsedTime");
+  });
+}
+
+usx_status usx_ctx_profile(usx_ctx* c, int enable) {
+  return guarded(c, [&] {
+    drain_profile(c);
+    c->totals.clear();
+    c->profiling = enable != 0;
+  });
+}
+
+usx_status usx_ctx_profile_report(usx_ctx* c, char* buf, size_t len) {
+  return guarded(c, [&] {
+    if (!buf || len == 0) fail(c, USX_ERR_INVALID, "profile report needs a buffer");
+    drain_profile(c);
+    std::string s = "{";
+    bool first = true;
+    for (const auto& t : c->totals) {
+      char item[256];
+      snprintf(item, sizeof(item), "%s\"%s\": [%ld, %.6f]", first ? "" : ", ", t.first.c_str(),
+               t.second.first, t.second.second);
+      s += item;
+      first = false;
+    }
+    s += "}";
+    if (s.size() + 1 > len) fail(c, USX_ERR_INVALID, "profile report buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+const char* usx_build_info(void) {
+  return "usx_hip gfx950 -O3 -ffp-contract=off correctly-rounded fp32 div/sqrt";
+}
+
+usx_status usx_demodulate(usx_ctx* c, const usx_demod_desc* d, const float* rf, usx_c32* iq,
+                          int memory) {
+  return guarded(c, [&] {
+    check_demod(c, d);
+    if (!rf || !iq) fail(c, USX_ERR_INVALID, "usx_demodulate: null data pointer");
+    const size_t nin = static_cast<size_t>(d->n_traces) * d->n_in;
+    const size_t nout = static_cast<size_t>(d->n_traces) * d->n_out;
+    if (memory == USX_DEVICE) {
+      run_demod(c, d, rf, reinterpret_cast<float2*>(iq));
+      check_launch(c, "demodulate launch");
+      return;
+    }
+    float* d_rf = static_cast<float*>(ensure(c, c->in, nin * sizeof(float)));
+    float2* d_iq = static_cast<float2*>(ensure(c, c->iq, nout * sizeof(float2)));
+    hip_check(c, hipMemcpyAsync(d_rf, rf, nin * sizeof(float), hipMemcpyHostToDevice, c->stream),
+              "hipMemcpyAsync(rf)");
+    run_demod(c, d, d_rf, d_iq);
+    check_launch(c, "demodulate launch");
+    hip_check(c, hipMemcpyAsync(iq, d_iq, nout * sizeof(float2), hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpyAsync(iq)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_beamform(usx_ctx* c, const usx_bf_desc* b, const void* data, usx_c32* frame,
+                        int memory) {
+  return guarded(c, [&] {
+    check_bf(c, b);
+    if (!data || !frame) fail(c, USX_ERR_INVALID, "usx_beamform: null data pointer");
+    const size_t npix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+    if (memory == USX_DEVICE) {
+      run_beamform(c, b, data, reinterpret_cast<float2*>(frame));
+      check_launch(c, "beamform launch");
+      return;
+    }
+    void* d_in = ensure(c, c->in, data_bytes(b));
+    float2* d_out = static_cast<float2*>(ensure(c, c->out, npix * sizeof(float2)));
+    hip_check(c, hipMemcpyAsync(d_in, data, data_bytes(b), hipMemcpyHostToDevice, c->stream),
+              "hipMemcpyAsync(data)");
+    run_beamform(c, b, d_in, d_out);
+    check_launch(c, "beamform launch");
+    hip_check(c, hipMemcpyAsync(frame, d_out, npix * sizeof(float2), hipMemcpyDeviceToHost,
+                                c->stream),
+              "hipMemcpyAsync(frame)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_process_frame(usx_ctx* c, const usx_demod_desc* d, const usx_bf_desc* b,
+                             const float* rf, usx_c32* frame, int memory, usx_stage_times* t) {
+  return guarded(c, [&] {
+    check_demod(c, d);
+    check_bf(c, b);
+    if (!rf || !frame) fail(c, USX_ERR_INVALID, "usx_process_frame: null data pointer");
+    if (!b->is_iq) fail(c, USX_ERR_INVALID, "usx_process_frame: beamform descriptor must describe IQ");
+    if (b->n_events * b->n_channels != d->n_traces || b->n_samples != d->n_out)
+      fail(c, USX_ERR_INVALID, "usx_process_frame: demodulate and beamform shapes disagree");
+    const size_t nin = static_cast<size_t>(d->n_traces) * d->n_in;
+    const size_t nout = static_cast<size_t>(d->n_traces) * d->n_out;
+    const size_t npix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+    const auto w0 = std::chrono::steady_clock::now();
+    hip_check(c, hipEventRecord(c->ev[0], c->stream), "hipEventRecord");
+    const float* d_rf = rf;
+    float2* d_out = reinterpret_cast<float2*>(frame);
+    if (memory != USX_DEVICE) {
+      float* p = static_cast<float*>(ensure(c, c->in, nin * sizeof(float)));
+      hip_check(c, hipMemcpyAsync(p, rf, nin * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                "hipMemcpyAsync(rf)");
+      d_rf = p;
+      d_out = static_cast<float2*>(ensure(c, c->out, npix * sizeof(float2)));
+    }
+    float2* d_iq = static_cast<float2*>(ensure(c, c->iq, nout * sizeof(float2)));
+    run_demod(c, d, d_rf, d_iq);
+    check_launch(c, "demodulate launch");
+    hip_check(c, hipEventRecord(c->ev[1], c->stream), "hipEventRecord");
+    run_beamform(c, b, d_iq, d_out);
+    check_launch(c, "beamform launch");
+    if (memory != USX_DEVICE)
+      hip_check(c, hipMemcpyAsync(frame, d_out, npix * sizeof(float2), hipMemcpyDeviceToHost,
+                                  c->stream),
+                "hipMemcpyAsync(frame)");
+    hip_check(c, hipEventRecord(c->ev[2], c->stream), "hipEventRecord");
+    if (memory != USX_DEVICE || t) {
+      hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    if (t) {
+      float a = 0, bb = 0;
+      hip_check(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "hipEventElapsedTime");
+      hip_check(c, hipEventElapsedTime(&bb, c->ev[1], c->ev[2]), "hipEventElapsedTime");
+      t->demodulate_ms = a;
+      t->beamform_ms = bb;
+      t->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
+  });
+}
+
+}  // extern "C"
