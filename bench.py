@@ -1,0 +1,238 @@
+"""Benchmark of the MI355X imaging path (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+
+Workload (one step = one frame through the hot path, RF already in HBM):
+    128-element 5 MHz linear array, 11 plane waves steered -9..9 deg,
+    RF float32 (11, 128, 1536) at 20 MHz from the point-scatterer simulator
+    (seed 0: 20k scatterers in x [-19, 19] mm, z [5, 50] mm, anechoic 3 mm
+    cyst at (0, 20 mm)); demodulation with decimation 4; minimum-variance
+    beamforming (subarray 64, diagonal loading 1e-2, +-2 depth averaging,
+    f/1.5) onto a 256 x 640 grid; complex64 frame out.
+The headline `value` is MV frames/s. DAS, DAS + coherence factor and DMAS
+frame rates on the same RF, and the end-to-end (host RF in, host frame out)
+MV rate, are reported under "extras".
+
+roofline: the dominant kernel of the MV step (`mv`, covariance + solve) is
+priced against the FP32 MFMA peak with the reference's FLOP count
+(8 sum n_snap L(L+1)/2 + 8 sum (L^3/6 + L^2) over pixels), divided by the
+kernel's mean duration from HIP events recorded on the library's stream
+over the timed region.
+
+cpu_baseline: the reference's own core (oracle/_ref, built from its sources)
+or else this repository's restatement (oracle/), timed on this host's cores
+on a bounded sample (64 lateral lines of the same depth range), scaled to a
+full frame.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ultrasound-engine_amd"))
+
+METRIC = "frames/s @256x640x11PW: DAS, MV, learned-MV at matched gCNR; % HBM roofline"
+PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_workload(usx):
+    """The seed-0 RF of the benchmark configuration and its grid."""
+    probe = usx.presets.linear_5mhz()
+    med = usx.Medium()
+    acq = usx.Acquisition()
+    acq.sampling_frequency = 20e6
+    acq.n_samples = 1536
+    acq.prf = 5000.0
+    sim = usx.Simulator(probe, med, acq, usx.SimulatorConfig())
+    ph = usx.Phantom.speckle_box(-0.019, 0.019, 0.005, 0.050, 2e-4, 20000, 0, 1.0)
+    ph.scale_amplitude_in_circle(usx.Vec3(0.0, 0.0, 0.020), 0.003, 0.0)
+    txs = sim.make_plane_waves(usx.angles_deg(-9, 9, 11))
+    raw = sim.simulate(txs, ph)
+    grid = usx.grid_for(probe, 0.005, 0.050, 256, 640)
+    return probe, med, acq, txs, ph, raw, grid
+
+
+def pipeline_config(usx, combiner, cf=False, subarray=64):
+    pc = usx.PipelineConfig()
+    pc.demod.decimation = 4
+    b = pc.beamformer
+    b.f_number = 1.5
+    b.combiner = getattr(usx.Combiner, combiner)
+    b.coherence_factor = cf
+    b.mv_subarray = subarray
+    b.mv_diagonal_loading = 1e-2
+    b.mv_range_average = 2
+    pc.beamformer = b
+    return pc
+
+
+def mv_reference_flops(probe, grid, f_number=1.5, subarray=64, range_avg=2):
+    """The reference's MV work per frame: covariance upper triangle
+    8 * n_snap * L(L+1)/2 plus Cholesky and solves 8 * (L^3/6 + L^2), per
+    pixel, with the reference's active-set, subarray and depth-window rules
+    (beamformer.cpp:142-173). Active channels: |x_e - x_p| <= z / (2 F#)."""
+    ex = np.asarray(probe.element_x, dtype=np.float64)
+    x = np.asarray(grid.axis0, dtype=np.float64)
+    z = np.asarray(grid.axis1, dtype=np.float64)
+    half = 0.5 * z / f_number
+    na = (np.abs(ex[None, None, :] - x[:, None, None]) < half[None, :, None]).sum(-1)
+    L = np.clip(np.minimum(subarray, np.maximum(2, na // 2)), 2, None)
+    ns = na - L + 1
+    n1 = z.size
+    j = np.arange(n1)
+    rows = (np.minimum(j + range_avg, n1 - 1) - np.maximum(j - range_avg, 0) + 1)[None, :]
+    live = na >= 4
+    cov = 8.0 * rows * ns * L * (L + 1) / 2.0
+    sol = 8.0 * (L ** 3 / 6.0 + L ** 2)
+    return float(np.sum(np.where(live, cov + sol, 0.0)))
+
+
+def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False):
+    """Device-resident frames/s: RF in HBM before the timed region; the
+    timed region is bracketed by synchronizations on both sides."""
+    runner = usx.DeviceRunner(grid, pc, raw)
+    rf = torch.from_numpy(np.ascontiguousarray(np.asarray(raw.rf))).to("cuda:0")
+    frame = torch.empty((grid.n_axis0, grid.n_axis1), dtype=torch.complex64, device="cuda:0")
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        runner.run(rf.data_ptr(), frame.data_ptr())
+    runner.synchronize()
+    if profile:
+        runner.profile(True)
+    torch.cuda.synchronize()
+    runner.synchronize()
+    t0 = time.perf_counter()
+    runner.record(0)
+    for _ in range(steps):
+        runner.run(rf.data_ptr(), frame.data_ptr())
+    runner.record(1)
+    runner.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = runner.elapsed(0, 1)
+    kern = json.loads(runner.profile_report()) if profile else {}
+    if profile:
+        runner.profile(False)
+    out = frame.cpu().numpy()
+    return wall * 1e3 / steps, ev_ms / steps, kern, out
+
+
+def time_host(usx, raw, grid, pc, steps):
+    """End-to-end: host ChannelData in, host Frame out (PCIe included)."""
+    pipe = usx.ImagingPipeline(grid, pc)
+    pipe.process_frame(raw)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.process_frame(raw)
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def cpu_baseline(usx, probe, med, acq, txs, raw, n_lines=128):
+    """Reference CPU MV on a bounded sample (n_lines lateral lines over the
+    full width and depth range), scaled to the 256-line frame."""
+    from oracle import oracle as O
+
+    flavour = "reference" if O.available("reference") else "port"
+    orc = O.load(flavour)
+    s = O.Setup.of(probe, med, acq)
+    otx = [O.Tx.of(t) for t in txs]
+    half = 0.5 * probe.aperture_width()
+    gs = O.GridSpec(0, -half, half, n_lines, 0.005, 0.050, 640)
+    bf = O.Bf(f_number=1.5, combiner=2, mv_subarray=64, mv_diagonal_loading=1e-2,
+              mv_range_average=2)
+    rf = np.asarray(raw.rf)
+    t0 = time.perf_counter()
+    _, st = orc.process_frame(s, otx, rf, gs, O.Demod(decimation=4), bf)
+    sample_s = time.perf_counter() - t0
+    frame_s = sample_s * (256.0 / n_lines)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": flavour,
+            "sample": f"MV frame on {n_lines} of 256 lateral lines x 640 depths "
+                      f"({sample_s:.1f} s), scaled x{256 // n_lines} to the full frame",
+            "sample_seconds": round(sample_s, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description="MI355X imaging-path benchmark")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; the imaging path is "
+                                   "data-parallel over frames and is measured per GPU",
+                          "n_gpus": a.gpus}))
+        return 2
+
+    import torch
+    import usx
+
+    log("simulating the benchmark RF (host) ...")
+    t = time.perf_counter()
+    probe, med, acq, txs, ph, raw, grid = make_workload(usx)
+    log(f"  RF {raw.shape} in {time.perf_counter() - t:.1f} s")
+
+    pc_mv = pipeline_config(usx, "MV")
+    log("MV device-resident ...")
+    mv_ms, mv_ev_ms, kern, mv_frame = time_device(usx, torch, raw, grid, pc_mv, a.steps,
+                                                  a.warmup, profile=True)
+    log(f"  {mv_ms:.3f} ms/frame (events {mv_ev_ms:.3f}), kernels {kern}")
+    extras = {"mv_fps": 1e3 / mv_ms}
+    for name, comb, cf in [("das", "DAS", False), ("das_cf", "DAS", True), ("dmas", "DMAS", False)]:
+        ms, ev_ms, k, _ = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
+                                      max(a.steps, 100), a.warmup, profile=True)
+        extras[f"{name}_fps"] = 1e3 / ms
+        extras[f"{name}_kernels_ms"] = {n: v[1] / v[0] for n, v in k.items()}
+        log(f"  {name}: {ms:.4f} ms/frame, kernels {k}")
+    extras["mv_end_to_end_fps"] = 1e3 / time_host(usx, raw, grid, pc_mv, max(3, a.steps // 4))
+    extras["mv_kernels_ms"] = {n: v[1] / v[0] for n, v in kern.items()}
+
+    flops = mv_reference_flops(probe, grid)
+    mv_kernel_ms = kern["mv"][1] / kern["mv"][0]
+    achieved = flops / (mv_kernel_ms * 1e-3)
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA, "traffic": None,
+            "kernel": "mv", "flops_per_launch": flops, "kernel_ms": mv_kernel_ms}
+    # DAS is priced on compulsory HBM bytes (RF in + frame out) for reference.
+    das_bytes = raw.rf.nbytes + grid.n_pixels * 8
+    das_total_ms = sum(extras["das_kernels_ms"].values())
+    extras["das_hbm_frac"] = das_bytes / (das_total_ms * 1e-3) / PEAK_HBM
+
+    cpu = None
+    if not a.skip_cpu:
+        log("CPU baseline (bounded sample) ...")
+        cpu = cpu_baseline(usx, probe, med, acq, txs, raw)
+        log(f"  {cpu}")
+        extras["mv_speedup_vs_cpu"] = (1e3 / mv_ms) / cpu["value"]
+
+    line = {
+        "metric": METRIC, "value": 1e3 / mv_ms, "unit": "frames/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": mv_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seed-0 point-scatterer RF from the host simulator)",
+        "config": {"workload": "MV process_frame: RF f32 (11,128,1536) @20 MHz, demod dec 4, "
+                               "MV L=64 loading 1e-2 range +-2, f/1.5, 256x640 grid",
+                   "grid": [256, 640], "channels": 128, "transmits": 11,
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "extras": extras,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -612,6 +612,11 @@ void beamform(const oc_setup& s, const oc_transmit* txs, int n_tx, bool is_iq, c
             }
             continue;
           }
+          if (ex.info) {
+            ex.info[3 * pix] = na;
+            ex.info[3 * pix + 1] = 0;
+            ex.info[3 * pix + 2] = 0;
+          }
           cf32 r(0, 0);
           if (bf.combiner == 0) {
             for (int ch = 0; ch < n_ch; ++ch) r += ybuf[ch];

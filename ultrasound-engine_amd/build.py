@@ -110,9 +110,9 @@ def build_module(force=True):
 
 
 def build_oracle():
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "port"], quiet=True)
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "port"], quiet=True)
     if os.path.isdir("/root/reference/core/src"):
-        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], quiet=True)
+        _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "ref"], quiet=True)
 
 
 def build_all(force=True):
