@@ -113,10 +113,10 @@ __device__ __forceinline__ float2 sample(const void* data, size_t base, int n, f
   if (IQ) {
     const float2* t = static_cast<const float2*>(data) + base;
     const float2 a = t[k], b = t[k + 1];
-    return make_float2(a.x * g + b.x * f, a.y * g + b.y * f);
+    return make_float2(fmaf(b.x, f, a.x * g), fmaf(b.y, f, a.y * g));
   }
   const float* t = static_cast<const float*>(data) + base;
-  return make_float2(t[k] * g + t[k + 1] * f, 0.0f);
+  return make_float2(fmaf(t[k + 1], f, t[k] * g), 0.0f);
 }
 
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
@@ -191,8 +191,8 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
         const float idx = (sl[s].tau + trx - A.t0) * A.fs;
         const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
         const float2 v = sample<IQ>(data, base, A.n_s, idx);
-        acc.x += v.x * sl[s].tr - v.y * sl[s].ti;
-        acc.y += v.x * sl[s].ti + v.y * sl[s].tr;
+        acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
+        acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
       }
     }
     float rr = a, ri = 0.0f;
@@ -302,8 +302,8 @@ __global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* 
           const float idx = (sl[s].tau + trx - A.t0) * A.fs;
           const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
           const float2 v = sample<IQ>(data, base, A.n_s, idx);
-          acc.x += v.x * sl[s].tr - v.y * sl[s].ti;
-          acc.y += v.x * sl[s].ti + v.y * sl[s].tr;
+          acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
+          acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
         }
       }
       float rr = a, ri = 0.0f;

@@ -52,7 +52,7 @@ struct usx_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t user_ev[8] = {};
   std::string err;
-  DevBuf in, out, iq, Y, mask, acc;
+  DevBuf in, out, iq, Y, mask, acc, mv_counts, mv_lists;
   Cached taps, rot, ex, tx, ax0, ax1, s0, c0;
   // profiling: pending (name, start, stop) event triples and the totals
   bool profiling = false;
@@ -165,15 +165,16 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
     fail(c, USX_ERR_INVALID, "IQ ChannelData needs a positive demodulation_frequency");
   if (b->mv_precision < 0 || b->mv_precision > 2) fail(c, USX_ERR_INVALID, "unknown mv_precision");
   if (b->combiner == 2) {
-    const int lmax = std::min(std::max(b->mv_subarray, 2), std::max(2, b->n_channels / 2));
-    const int pmax = (lmax + 15) & ~15;
-    const size_t lds = static_cast<size_t>(b->n_channels) * 4 + 16 +
-                       8 * (static_cast<size_t>(2 * b->mv_range_average + 1) * b->n_channels +
-                            static_cast<size_t>(pmax) * pmax + 3 * pmax);
+    const int pmax = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray);
+    if (pmax > usxd::kMvMaxP)
+      fail(c, USX_ERR_INVALID,
+           "MV: subarray length up to %d with %d channels exceeds the supported %d",
+           std::min(b->mv_subarray, b->n_channels / 2), b->n_channels, usxd::kMvMaxP);
+    const size_t lds = usxd::mv_smem_bytes(b->n_channels, b->mv_range_average, b->mv_subarray, pmax);
     if (lds > 65536)
       fail(c, USX_ERR_INVALID,
-           "MV: subarray %d with %d channels and range average %d needs %zu B of LDS per pixel "
-           "(limit 65536)", lmax, b->n_channels, b->mv_range_average, lds);
+           "MV: %d channels with range average %d need %zu B of LDS per pixel (limit 65536)",
+           b->n_channels, b->mv_range_average, lds);
   }
 }
 
@@ -288,6 +289,12 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
   M.cf = cf;
   M.precision = b->mv_precision;
   M.refine = b->mv_refine;
+  auto mv = [&](float2* Y, uint32_t* mk, float* acc) {
+    const int nb = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray) / 16;
+    int* counts = static_cast<int*>(ensure(c, c->mv_counts, 16 * sizeof(int)));
+    int* lists = static_cast<int*>(ensure(c, c->mv_lists, static_cast<size_t>(nb) * n_pix * sizeof(int)));
+    timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, counts, lists, st); });
+  };
 
   if (incoh) {
     float* acc = static_cast<float*>(ensure(c, c->acc, n_pix * sizeof(float)));
@@ -298,7 +305,7 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
       timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e, 1, 0, 1, comb == 2, iq, st); });
       if (comb == 2) {
         M.incoherent = 1;
-        timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, st); });
+        mv(Y, mk, acc);
       } else {
         timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, acc, 1, comb, cf, st); });
       }
@@ -319,7 +326,7 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
       }
     }
     M.incoherent = 0;
-    timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, nullptr, st); });
+    mv(Y, mk, nullptr);
     return;
   }
 
@@ -406,7 +413,7 @@ usx_status usx_ctx_destroy(usx_ctx* c) {
   }
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->in, &c->out, &c->iq, &c->Y, &c->mask, &c->acc})
+  for (DevBuf* b : {&c->in, &c->out, &c->iq, &c->Y, &c->mask, &c->acc, &c->mv_counts, &c->mv_lists})
     if (b->p) (void)hipFree(b->p);
   for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0})
     if (t->dev.p) (void)hipFree(t->dev.p);

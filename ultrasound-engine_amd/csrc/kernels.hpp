@@ -54,7 +54,12 @@ struct MvArgs {
   int precision;   // 0 FP32, 1 BF16, 2 FP16 covariance inputs
   int refine;      // one step of iterative refinement (FP64 residual)
 };
+// counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
-               hipStream_t st);
+               int* counts, int* lists, hipStream_t st);
+// Largest covariance size bucket (L rounded up to 16) a configuration can produce.
+int mv_max_bucket_p(int n_ch, int subarray);
+size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P);
+constexpr int kMvMaxP = 64;
 
 }  // namespace usxd

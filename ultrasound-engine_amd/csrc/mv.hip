@@ -3,29 +3,35 @@
 // Reference: combine_mv_windowed and cholesky_solve,
 // core/src/beamformer.cpp:47-79, 137-214; coherence factor :525-534.
 //
-// One wavefront per pixel:
-//   1. active list of the centre pixel (bit mask from the channel kernel),
-//      L = clamp(subarray, 2, max(2, n/2)), n_sub = n - L + 1;
-//      fewer than 4 active channels -> DAS sum (beamformer.cpp:142-146);
-//   2. the channel vectors of the centre depth and its +-range_avg neighbours
-//      (clipped at the grid edge) are staged in LDS;
-//   3. covariance R = sum over (depth, subarray start) of x x^H as a dense
+// Two stages per frame:
+//
+// mv_classify (one thread per pixel): active count n from the channel mask;
+//   n == 0 -> 0; n < 4 -> the DAS sum (beamformer.cpp:142-146), with the
+//   coherence factor; otherwise L = clamp(subarray, 2, max(2, n/2)) and the
+//   pixel is appended to the list of its size bucket P = 16 ceil(L/16).
+//
+// mv_solve<P> (persistent, one wavefront per pixel, LDS sized for the
+// bucket so small systems run at high occupancy):
+//   1. active list; centre and +-range_avg neighbour vectors staged in LDS
+//      (neighbours clipped at the grid edge, beamformer.cpp:159-161);
+//   2. covariance R = sum over (depth, subarray start) of x x^H as a dense
 //      rank-k update on MFMA, complex expanded to real:
 //        Re R = [xr xi][xr xi]^T,  Im R = [xi -xr][xr xi]^T,
 //      v_mfma_f32_16x16x4_f32, two snapshots per instruction (k = re/im of
-//      snapshot t, re/im of snapshot t+1), L padded to a multiple of 16 with
-//      zero rows; only upper-triangle 16x16 blocks are formed;
-//   4. R /= n_snap, diagonal loading max(delta * tr(R) / L, 1e-20) with the
-//      trace summed in FP64;
-//   5. in-place FP32 Cholesky in LDS (the strictly upper triangle keeps R
-//      for the refinement residual), forward/back substitution for R w = 1;
-//      failure (non-positive pivot) -> DAS sum (beamformer.cpp:199);
-//   6. optional refinement: r = 1 - R w in FP64, R d = r, w += d;
-//   7. |sum w| < 1e-30 -> DAS sum; out = (w^H avg) / sum w * n_active with
-//      avg the subarray average of the centre vector (beamformer.cpp:205-213);
-//   8. coherence factor over the centre vector, or |out|^2 accumulation for
-//      incoherent compounding.
-// precision 1 / 2 rounds the MFMA inputs to bf16 / fp16 (FP32 accumulate),
+//      snapshot t, re/im of snapshot t+1), L padded to P with zero rows,
+//      upper-triangle 16x16 blocks only;
+//   3. R /= n_snap; diagonal loading max(delta tr(R)/L, 1e-20) with the trace
+//      summed in FP64;
+//   4. in-place FP32 Cholesky in LDS, element-parallel right-looking updates
+//      (the strictly upper triangle keeps R for the residual); a non-positive
+//      pivot -> DAS sum (beamformer.cpp:199);
+//   5. triangular solves for R w = 1 with the vector in registers (lane i
+//      holds w_i) and the pivot broadcast by readlane: no barriers;
+//   6. optional single refinement step: r = 1 - R w in FP64, R d = r, w += d;
+//   7. |sum w| < 1e-30 -> DAS; out = (w^H avg) / sum w * n with avg the
+//      subarray average of the centre vector (beamformer.cpp:202-213);
+//   8. coherence factor, or |out|^2 accumulation for incoherent compounding.
+// precision 1 / 2 rounds the MFMA inputs to bf16 / fp16 (FP32 accumulate):
 // the reported precision variants; 0 (FP32 inputs) is the parity path.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -51,6 +57,9 @@ __device__ __forceinline__ double wsum_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
 
 template <int PREC>
 __device__ __forceinline__ float quant(float x) {
@@ -59,348 +68,447 @@ __device__ __forceinline__ float quant(float x) {
   return x;
 }
 
-struct MvSmem {
-  int* act;        // n_ch
-  float2* rows;    // nrow_max * n_ch
-  float2* A;       // P * P : strictly upper = R, lower + diag = Cholesky factor
-  float2* dg;      // P : loaded diagonal of R (real)
-  float2* w;       // P
-  float2* r;       // P
-};
+constexpr int kMaxBuckets = 16;  // P up to 256
 
-constexpr int kPairsPerGroup = 4;
+// Coherence factor |sum y|^2 / (n sum |y|^2) over the active centre vector.
+__device__ __forceinline__ float coherence(const float2* yc, const int* act, int na, int lane) {
+  float cr = 0.0f, ci = 0.0f, inc = 0.0f;
+  for (int a = lane; a < na; a += 64) {
+    const float2 v = yc[act[a]];
+    cr += v.x;
+    ci += v.y;
+    const float m = hypotf(v.x, v.y);
+    inc += m * m;
+  }
+  cr = wsum(cr);
+  ci = wsum(ci);
+  inc = wsum(inc);
+  const float mc = hypotf(cr, ci);
+  return (inc > 0.0f) ? (mc * mc) / (static_cast<float>(na) * inc) : 0.0f;
+}
+
+__device__ __forceinline__ void store_out(const MvArgs& M, int pix, float2 out, float2* frame,
+                                          float* acc) {
+  if (M.incoherent) {
+    const float m = hypotf(out.x, out.y);
+    acc[pix] += m * m;
+  } else {
+    frame[pix] = out;
+  }
+}
 
 }  // namespace
 
-size_t mv_smem_bytes(int n_ch, int range_avg, int pmax) {
+// Bytes of dynamic LDS for one wavefront of bucket P.
+size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
+  const int na_max = subarray <= P ? n_ch : std::min(n_ch, 2 * P + 1);
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * n_ch +
-                                  static_cast<size_t>(pmax) * pmax + 3 * static_cast<size_t>(pmax));
+  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max +
+                                  static_cast<size_t>(P) * (P + 1) + 2 * static_cast<size_t>(P));
 }
 
-template <int PREC>
-__global__ __launch_bounds__(64) void mv_kernel(MvArgs M, const float2* __restrict__ Y,
-                                                const uint32_t* __restrict__ mask,
-                                                float2* __restrict__ frame, float* acc_out,
-                                                int pmax) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int pix = blockIdx.x;
-  const int i0 = pix / M.n1;
-  const int j = pix - i0 * M.n1;
-  const int lane = threadIdx.x;
-  const int n_ch = M.n_ch;
-  const int nw = (n_ch + 31) / 32;
-  const int nrow_max = 2 * M.range_avg + 1;
+__global__ void mv_zero_counts(int* counts) {
+  if (threadIdx.x < kMaxBuckets) counts[threadIdx.x] = 0;
+}
 
-  MvSmem S;
-  S.act = reinterpret_cast<int*>(smem_raw);
-  S.rows = reinterpret_cast<float2*>(smem_raw + (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16);
-  S.A = S.rows + static_cast<size_t>(nrow_max) * n_ch;
-  S.dg = S.A + static_cast<size_t>(pmax) * pmax;
-  S.w = S.dg + pmax;
-  S.r = S.w + pmax;
-
-  // ---- 1. active list of the centre pixel --------------------------------
-  const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
-  int na = 0;
-  for (int base = 0; base < n_ch; base += 64) {
-    const int ch = base + lane;
-    const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
-    const unsigned long long bal = __ballot(on);
-    const int pos = na + __popcll(bal & ((1ull << lane) - 1ull));
-    if (on) S.act[pos] = ch;
-    na += __popcll(bal);
+// Slot in bucket b's list, one atomic per (wavefront, bucket): lanes of a
+// wavefront that join the same bucket take consecutive slots.
+__device__ __forceinline__ int bucket_slot(int* counts, int b, bool take) {
+  int slot = 0;
+  for (int q = 0; q < kMaxBuckets; ++q) {
+    const unsigned long long m = __ballot(take && b == q);
+    if (m == 0) continue;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    int base = 0;
+    if (static_cast<int>(threadIdx.x & 63) == leader) base = atomicAdd(&counts[q], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (take && b == q)
+      slot = base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
   }
-  __syncthreads();
+  return slot;
+}
 
-  const float2* yc = Y + static_cast<size_t>(pix) * n_ch;
-  auto finish = [&](float2 out) {
-    if (M.cf) {
-      float cr = 0.0f, ci = 0.0f, inc = 0.0f;
-      for (int a = lane; a < na; a += 64) {
-        const float2 v = yc[S.act[a]];
-        cr += v.x;
-        ci += v.y;
+__global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __restrict__ Y,
+                                                   const uint32_t* __restrict__ mask,
+                                                   float2* __restrict__ frame, float* acc,
+                                                   int* counts, int* lists) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_pix = M.n0 * M.n1;
+  const bool valid = pix < n_pix;
+  const int nw = (M.n_ch + 31) / 32;
+  const uint32_t* mp = mask + static_cast<size_t>(valid ? pix : 0) * nw;
+  int na = 0;
+  if (valid)
+    for (int w = 0; w < nw; ++w) na += __popc(mp[w]);
+  if (valid && na == 0 && !M.incoherent) frame[pix] = make_float2(0.0f, 0.0f);
+  if (valid && na > 0 && na < 4) {
+    const float2* yc = Y + static_cast<size_t>(pix) * M.n_ch;
+    float sr = 0.0f, si = 0.0f, inc = 0.0f;
+    for (int w = 0; w < nw; ++w) {
+      uint32_t bits = mp[w];
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const float2 v = yc[w * 32 + b];
+        sr += v.x;
+        si += v.y;
         const float m = hypotf(v.x, v.y);
         inc += m * m;
       }
-      cr = wsum(cr);
-      ci = wsum(ci);
-      inc = wsum(inc);
-      const float mc = hypotf(cr, ci);
+    }
+    float2 out = make_float2(sr, si);
+    if (M.cf) {
+      const float mc = hypotf(sr, si);
       const float f = (inc > 0.0f) ? (mc * mc) / (static_cast<float>(na) * inc) : 0.0f;
       out.x *= f;
       out.y *= f;
     }
-    if (lane == 0) {
-      if (M.incoherent) {
-        const float m = hypotf(out.x, out.y);
-        acc_out[pix] += m * m;
-      } else {
-        frame[pix] = out;
-      }
-    }
-  };
-  auto das = [&]() {
-    float sr = 0.0f, si = 0.0f;
-    for (int a = lane; a < na; a += 64) {
-      const float2 v = yc[S.act[a]];
-      sr += v.x;
-      si += v.y;
-    }
-    return make_float2(wsum(sr), wsum(si));
-  };
-
-  if (na == 0) {
-    if (lane == 0 && !M.incoherent) frame[pix] = make_float2(0.0f, 0.0f);
-    return;
+    store_out(M, pix, out, frame, acc);
   }
-  if (na < 4) {
-    finish(das());
-    return;
-  }
+  const bool take = valid && na >= 4;
   const int L = min(max(M.subarray, 2), max(2, na / 2));
-  const int ns = na - L + 1;
-  const int P = (L + 15) & ~15;
-  const int nb = P / 16;
+  const int b = take ? (L + 15) / 16 - 1 : -1;
+  const int slot = bucket_slot(counts, b, take);
+  if (take) lists[static_cast<size_t>(b) * n_pix + slot] = pix;
+}
 
-  // ---- 2. stage centre and neighbour rows --------------------------------
-  int nrow = 0, crow = 0;
-  for (int d = -M.range_avg; d <= M.range_avg; ++d) {
-    const int k = j + d;
-    if (k < 0 || k >= M.n1) continue;
-    if (d == 0) crow = nrow;
-    const float2* src = Y + (static_cast<size_t>(i0) * M.n1 + k) * n_ch;
-    for (int a = lane; a < na; a += 64) S.rows[nrow * na + a] = src[S.act[a]];
-    ++nrow;
-  }
-  __syncthreads();
-  const int n_snap = nrow * ns;
+template <int P, int PREC>
+__global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restrict__ Y,
+                                               const uint32_t* __restrict__ mask,
+                                               float2* __restrict__ frame, float* acc,
+                                               const int* __restrict__ counts,
+                                               const int* __restrict__ lists, int na_max) {
+  constexpr int NB = P / 16;
+  constexpr int NPAIRS = NB * (NB + 1) / 2;
+  constexpr int BUCKET = P / 16 - 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x;
+  const int n_ch = M.n_ch;
+  const int nw = (n_ch + 31) / 32;
+  const int n_pix = M.n0 * M.n1;
+  int* act = reinterpret_cast<int*>(smem_raw);
+  float2* rows = reinterpret_cast<float2*>(smem_raw + (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16);
+  // A: P rows of stride P + 1 (conflict-free column access). Strictly upper
+  // triangle: R (kept for the residual); lower triangle + diagonal: the
+  // Cholesky factor. dg[a] = (loaded diagonal of R, 1 / L_aa).
+  constexpr int S = P + 1;
+  float2* A = rows + static_cast<size_t>(2 * M.range_avg + 1) * na_max;
+  float2* dg = A + P * S;
+  float2* rv = dg + P;  // scratch vector
 
-  // ---- 3. covariance on MFMA ---------------------------------------------
-  const int li = lane & 15, kk = lane >> 4;
-  const int tsub = kk >> 1, comp = kk & 1;
-  const int n_pairs = nb * (nb + 1) / 2;
-  const float inv = 1.0f / static_cast<float>(n_snap);
-  for (int p0 = 0; p0 < n_pairs; p0 += kPairsPerGroup) {
-    f4 accR[kPairsPerGroup], accI[kPairsPerGroup];
-    int pbi[kPairsPerGroup], pbj[kPairsPerGroup];
-#pragma unroll
-    for (int q = 0; q < kPairsPerGroup; ++q) {
-      accR[q] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-      accI[q] = accR[q];
-      // decode pair index p0+q -> (bi <= bj), row-major over the upper triangle
-      int p = p0 + q, bi = 0;
-      while (p >= nb - bi && bi < nb) {
-        p -= nb - bi;
-        ++bi;
-      }
-      pbi[q] = bi;
-      pbj[q] = bi + p;
+  const int count = counts[BUCKET];
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const int pix = lists[static_cast<size_t>(BUCKET) * n_pix + item];
+    const int i0 = pix / M.n1;
+    const int j = pix - i0 * M.n1;
+    __syncthreads();  // LDS of the previous pixel is free
+
+    // ---- 1. active list ------------------------------------------------------
+    const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
+    int na = 0;
+    for (int base = 0; base < n_ch; base += 64) {
+      const int ch = base + lane;
+      const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
+      const unsigned long long bal = __ballot(on);
+      if (on) act[na + __popcll(bal & ((1ull << lane) - 1ull))] = ch;
+      na += __popcll(bal);
     }
-    for (int t = 0; t < n_snap; t += 2) {
-      const int ts = t + tsub;
-      const bool tv = ts < n_snap;
-      const int ri = tv ? ts / ns : 0;
-      const int s0 = tv ? ts - ri * ns : 0;
-      const float2* xrow = S.rows + ri * na + s0;
+    __syncthreads();
+    const float2* yc = Y + static_cast<size_t>(pix) * n_ch;
+    const int L = min(max(M.subarray, 2), max(2, na / 2));
+    const int ns = na - L + 1;
+
+    // ---- 2. stage centre and neighbour rows ---------------------------------
+    int nrow = 0, crow = 0;
+    for (int d = -M.range_avg; d <= M.range_avg; ++d) {
+      const int k = j + d;
+      if (k < 0 || k >= M.n1) continue;
+      if (d == 0) crow = nrow;
+      const float2* src = Y + (static_cast<size_t>(i0) * M.n1 + k) * n_ch;
+      for (int a = lane; a < na; a += 64) rows[nrow * na + a] = src[act[a]];
+      ++nrow;
+    }
+    __syncthreads();
+    const int n_snap = nrow * ns;
+
+    // ---- 3. covariance on MFMA ----------------------------------------------
+    const int li = lane & 15, kk = lane >> 4;
+    const int tsub = kk >> 1, comp = kk & 1;
+    // FP16 inputs: scale by a power of two so the largest component sits
+    // near 2^8 (half overflows at 65504); the scale leaves relative rounding
+    // unchanged and is divided back out of R exactly.
+    float sc = 1.0f, unsc2 = 1.0f;
+    if (PREC == 2) {
+      float mx = 0.0f;
+      for (int q = lane; q < nrow * na; q += 64)
+        mx = fmaxf(mx, fmaxf(fabsf(rows[q].x), fabsf(rows[q].y)));
 #pragma unroll
-      for (int q = 0; q < kPairsPerGroup; ++q) {
-        if (p0 + q < n_pairs) {
-          const int ai = pbi[q] * 16 + li, aj = pbj[q] * 16 + li;
-          float2 xi2 = make_float2(0.0f, 0.0f), xj2 = xi2;
-          if (tv && ai < L) xi2 = xrow[ai];
-          if (tv && aj < L) xj2 = xrow[aj];
-          const float ar = quant<PREC>(comp ? xi2.y : xi2.x);
-          const float aim = quant<PREC>(comp ? -xi2.x : xi2.y);
-          const float b = quant<PREC>(comp ? xj2.y : xj2.x);
-          accR[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b, accR[q], 0, 0, 0);
-          accI[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, accI[q], 0, 0, 0);
+      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      if (mx > 0.0f) {
+        const int e = ilogbf(mx) - 8;
+        sc = ldexpf(1.0f, -e);
+        unsc2 = ldexpf(1.0f, 2 * e);
+      }
+    }
+    const float inv = unsc2 / static_cast<float>(n_snap);
+    {
+      // All upper-triangle block pairs in one pass: per snapshot pair, one
+      // operand load per 16-row block, then 2 MFMAs per block pair.
+      f4 accR[NPAIRS], accI[NPAIRS];
+#pragma unroll
+      for (int q = 0; q < NPAIRS; ++q) {
+        accR[q] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        accI[q] = accR[q];
+      }
+      bool rowok[NB];
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) rowok[bb] = bb * 16 + li < L;
+      int ri = 0, s0 = tsub;  // snapshot t + tsub as (row, subarray start)
+      if (s0 >= ns) {
+        s0 -= ns;
+        ++ri;
+      }
+      for (int t = 0; t < n_snap; t += 2) {
+        const bool tv = t + tsub < n_snap;
+        const float2* xrow = rows + ri * na + s0 + li;
+        float ar[NB], aim[NB];
+#pragma unroll
+        for (int bb = 0; bb < NB; ++bb) {
+          float2 x = make_float2(0.0f, 0.0f);
+          if (tv && rowok[bb]) x = xrow[bb * 16];
+          ar[bb] = quant<PREC>(sc * (comp ? x.y : x.x));
+          aim[bb] = quant<PREC>(sc * (comp ? -x.x : x.y));
+        }
+        int q = 0;
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi) {
+#pragma unroll
+          for (int bj = bi; bj < NB; ++bj, ++q) {
+            accR[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[bi], ar[bj], accR[q], 0, 0, 0);
+            accI[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[bi], ar[bj], accI[q], 0, 0, 0);
+          }
+        }
+        s0 += 2;
+        if (s0 >= ns) {
+          s0 -= ns;
+          ++ri;
+          if (s0 >= ns) {  // ns == 1: two rows per step
+            s0 -= ns;
+            ++ri;
+          }
         }
       }
-    }
-    // D[4*(lane/16) + e][lane%16] -> R[bi*16 + row][bj*16 + col]
+      // D[4*(lane/16) + e][lane%16] -> R[bi*16 + row][bj*16 + col]
+      int q = 0;
 #pragma unroll
-    for (int q = 0; q < kPairsPerGroup; ++q) {
-      if (p0 + q < n_pairs) {
+      for (int bi = 0; bi < NB; ++bi) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int row = pbi[q] * 16 + kk * 4 + e;
-          const int col = pbj[q] * 16 + li;
-          if (row < L && col < L && row <= col) {
-            const float2 v = make_float2(accR[q][e] * inv, accI[q][e] * inv);
-            if (row == col) {
-              S.dg[row] = make_float2(v.x, 0.0f);
-            } else {
-              S.A[row * P + col] = v;                           // R (upper)
-              S.A[col * P + row] = make_float2(v.x, -v.y);      // factor input (lower)
+        for (int bj = bi; bj < NB; ++bj, ++q) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = bi * 16 + kk * 4 + e;
+            const int col = bj * 16 + li;
+            if (row < L && col < L && row <= col) {
+              const float2 v = make_float2(accR[q][e] * inv, accI[q][e] * inv);
+              if (row == col) {
+                dg[row] = make_float2(v.x, 0.0f);
+              } else {
+                A[row * S + col] = v;                       // R, upper
+                A[col * S + row] = make_float2(v.x, -v.y);  // factor input, lower
+              }
             }
           }
         }
       }
     }
-  }
-  __syncthreads();
-
-  // ---- 4. diagonal loading (trace in FP64) -------------------------------
-  double tr = 0.0;
-  for (int a = lane; a < L; a += 64) tr += static_cast<double>(S.dg[a].x);
-  tr = wsum_d(tr);
-  const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
-  for (int a = lane; a < L; a += 64) {
-    const float d = S.dg[a].x + eps;
-    S.dg[a] = make_float2(d, 0.0f);
-    S.A[a * P + a] = make_float2(d, 0.0f);
-    S.w[a] = make_float2(1.0f, 0.0f);
-  }
-  __syncthreads();
-
-  // ---- 5. Cholesky (lower, in place) and solve ---------------------------
-  bool ok = true;
-  for (int k = 0; k < L; ++k) {
-    const float d = S.A[k * P + k].x;
-    if (!(d > 0.0f)) {
-      ok = false;
-      break;
-    }
-    const float lkk = sqrtf(d);
-    for (int i = k + 1 + lane; i < L; i += 64) {
-      const float2 v = S.A[i * P + k];
-      S.A[i * P + k] = make_float2(v.x / lkk, v.y / lkk);
-    }
-    if (lane == 0) S.A[k * P + k] = make_float2(lkk, 0.0f);
     __syncthreads();
-    const int m = L - k - 1;
-    for (int q = lane; q < m * m; q += 64) {
-      const int r = q / m, c = q - r * m;
-      if (c <= r) {
-        const int ii = k + 1 + r, jj = k + 1 + c;
-        const float2 lik = S.A[ii * P + k], ljk = S.A[jj * P + k];
-        float2 a = S.A[ii * P + jj];
-        // a -= lik * conj(ljk)
-        a.x -= lik.x * ljk.x + lik.y * ljk.y;
-        a.y -= lik.y * ljk.x - lik.x * ljk.y;
-        S.A[ii * P + jj] = a;
-      }
-    }
-    __syncthreads();
-  }
-  if (!ok) {
-    finish(das());
-    return;
-  }
 
-  auto solve = [&](float2* b) {
-    // forward: L y = b
-    for (int k = 0; k < L; ++k) {
-      const float lkk = S.A[k * P + k].x;
-      const float2 bk = b[k];
-      const float2 yk = make_float2(bk.x / lkk, bk.y / lkk);
-      __syncthreads();
-      if (lane == 0) b[k] = yk;
-      for (int i = k + 1 + lane; i < L; i += 64) {
-        const float2 l = S.A[i * P + k];
-        float2 v = b[i];
-        v.x -= l.x * yk.x - l.y * yk.y;
-        v.y -= l.x * yk.y + l.y * yk.x;
-        b[i] = v;
-      }
-      __syncthreads();
-    }
-    // back: L^H x = y
-    for (int k = L - 1; k >= 0; --k) {
-      const float lkk = S.A[k * P + k].x;
-      const float2 bk = b[k];
-      const float2 xk = make_float2(bk.x / lkk, bk.y / lkk);
-      __syncthreads();
-      if (lane == 0) b[k] = xk;
-      for (int i = lane; i < k; i += 64) {
-        const float2 l = S.A[k * P + i];  // conj(L[k][i]) * xk
-        float2 v = b[i];
-        v.x -= l.x * xk.x + l.y * xk.y;
-        v.y -= l.x * xk.y - l.y * xk.x;
-        b[i] = v;
-      }
-      __syncthreads();
-    }
-  };
-  solve(S.w);
-
-  // ---- 6. one refinement step, FP64 residual ------------------------------
-  if (M.refine) {
+    // ---- 4. diagonal loading, trace in FP64 -----------------------------------
+    double tr = 0.0;
+    for (int a = lane; a < L; a += 64) tr += static_cast<double>(dg[a].x);
+    tr = wsum_d(tr);
+    const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
     for (int a = lane; a < L; a += 64) {
-      double rr = 1.0, ri = 0.0;
-      const float2 wa = S.w[a];
-      rr -= static_cast<double>(S.dg[a].x) * wa.x;
-      ri -= static_cast<double>(S.dg[a].x) * wa.y;
-      for (int b = 0; b < L; ++b) {
-        if (b == a) continue;
-        // R[a][b]: upper stored at [min][max]; lower = conj
-        float2 rab = (a < b) ? S.A[a * P + b] : S.A[b * P + a];
-        if (a > b) rab.y = -rab.y;
-        const float2 wb = S.w[b];
-        rr -= static_cast<double>(rab.x) * wb.x - static_cast<double>(rab.y) * wb.y;
-        ri -= static_cast<double>(rab.x) * wb.y + static_cast<double>(rab.y) * wb.x;
-      }
-      S.r[a] = make_float2(static_cast<float>(rr), static_cast<float>(ri));
+      const float d = dg[a].x + eps;
+      dg[a] = make_float2(d, 0.0f);
+      A[a * S + a] = make_float2(d, 0.0f);
     }
     __syncthreads();
-    solve(S.r);
-    for (int a = lane; a < L; a += 64) {
-      float2 wa = S.w[a];
-      wa.x += S.r[a].x;
-      wa.y += S.r[a].y;
-      S.w[a] = wa;
-    }
-    __syncthreads();
-  }
 
-  // ---- 7. weights applied to the subarray-averaged centre snapshot -------
-  float dr = 0.0f, di = 0.0f, orr = 0.0f, oi = 0.0f;
-  const float2* cen = S.rows + crow * na;
-  for (int a = lane; a < L; a += 64) {
-    const float2 wa = S.w[a];
-    dr += wa.x;
-    di += wa.y;
-    float sr = 0.0f, si = 0.0f;
-    for (int s0 = 0; s0 < ns; ++s0) {
-      sr += cen[s0 + a].x;
-      si += cen[s0 + a].y;
+    // ---- 5. Cholesky, left-looking by columns: lane i owns row i -------------
+    //   L_ij = (A_ij - sum_{k<j} L_ik conj(L_jk)) / L_jj, L_jj = sqrt(A_jj - sum |L_jk|^2),
+    // the reference's sums (beamformer.cpp:48-63); one barrier per column.
+    bool ok = true;
+    for (int j = 0; j < L; ++j) {
+      float sr = 0.0f, si = 0.0f;
+      if (lane >= j && lane < L) {
+        const float2* ri_ = A + lane * S;
+        const float2* rj = A + j * S;
+        sr = ri_[j].x;
+        si = ri_[j].y;
+#pragma unroll 4
+        for (int k = 0; k < j; ++k) {
+          const float2 a = ri_[k], b = rj[k];  // -= a * conj(b)
+          sr = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr));
+          si = fmaf(-a.y, b.x, fmaf(a.x, b.y, si));
+        }
+      }
+      const float d = lane_bcast(sr, j);
+      if (!(d > 0.0f)) {
+        ok = false;
+        break;
+      }
+      const float ljj = sqrtf(d);
+      const float rj = 1.0f / ljj;
+      if (lane == j) {
+        A[j * S + j] = make_float2(ljj, 0.0f);
+        dg[j].y = rj;
+      } else if (lane > j && lane < L) {
+        A[lane * S + j] = make_float2(sr * rj, si * rj);
+      }
+      __syncthreads();
     }
-    sr /= static_cast<float>(ns);
-    si /= static_cast<float>(ns);
-    // conj(w) * avg
-    orr += wa.x * sr + wa.y * si;
-    oi += wa.x * si - wa.y * sr;
+    float2 out;
+    if (!ok) {
+      float sr = 0.0f, si = 0.0f;
+      for (int a = lane; a < na; a += 64) {
+        sr += yc[act[a]].x;
+        si += yc[act[a]].y;
+      }
+      out = make_float2(wsum(sr), wsum(si));
+    } else {
+      // ---- 6. solves with the vector in registers (lane i holds b_i) --------
+      auto solve = [&](float br, float bi) {
+        for (int k = 0; k < L; ++k) {  // L y = b
+          const float rk = dg[k].y;
+          const float yr = lane_bcast(br, k) * rk, yi = lane_bcast(bi, k) * rk;
+          if (lane == k) {
+            br = yr;
+            bi = yi;
+          } else if (lane > k && lane < L) {
+            const float2 l = A[lane * S + k];
+            br = fmaf(-l.x, yr, fmaf(l.y, yi, br));
+            bi = fmaf(-l.x, yi, fmaf(-l.y, yr, bi));
+          }
+        }
+        for (int k = L - 1; k >= 0; --k) {  // L^H x = y
+          const float rk = dg[k].y;
+          const float xr = lane_bcast(br, k) * rk, xi = lane_bcast(bi, k) * rk;
+          if (lane == k) {
+            br = xr;
+            bi = xi;
+          } else if (lane < k) {
+            const float2 l = A[k * S + lane];  // -= conj(L[k][lane]) * x_k
+            br = fmaf(-l.x, xr, fmaf(-l.y, xi, br));
+            bi = fmaf(-l.x, xi, fmaf(l.y, xr, bi));
+          }
+        }
+        return make_float2(br, bi);
+      };
+      float2 w = solve(lane < L ? 1.0f : 0.0f, 0.0f);
+
+      if (M.refine) {  // ---- 7. one refinement step, FP64 residual ----------
+        if (lane < P) rv[lane] = lane < L ? w : make_float2(0.0f, 0.0f);
+        __syncthreads();
+        float2 r = make_float2(0.0f, 0.0f);
+        if (lane < L) {
+          const int a = lane;
+          double rr = 1.0 - static_cast<double>(dg[a].x) * w.x;
+          double ri = -static_cast<double>(dg[a].x) * w.y;
+          for (int b = 0; b < L; ++b) {
+            if (b == a) continue;
+            float2 rab = (a < b) ? A[a * S + b] : A[b * S + a];
+            if (a > b) rab.y = -rab.y;
+            const float2 wb = rv[b];
+            rr = fma(-static_cast<double>(rab.x), wb.x, fma(static_cast<double>(rab.y), wb.y, rr));
+            ri = fma(-static_cast<double>(rab.x), wb.y, fma(-static_cast<double>(rab.y), wb.x, ri));
+          }
+          r = make_float2(static_cast<float>(rr), static_cast<float>(ri));
+        }
+        const float2 dlt = solve(r.x, r.y);
+        if (lane < L) {
+          w.x += dlt.x;
+          w.y += dlt.y;
+        }
+      }
+
+      // ---- 8. weights on the subarray-averaged centre snapshot --------------
+      float dr = 0.0f, di = 0.0f, orr = 0.0f, oi = 0.0f;
+      if (lane < L) {
+        const float2* cen = rows + crow * na;
+        float sr = 0.0f, si = 0.0f;
+        for (int s0 = 0; s0 < ns; ++s0) {
+          sr += cen[s0 + lane].x;
+          si += cen[s0 + lane].y;
+        }
+        sr /= static_cast<float>(ns);
+        si /= static_cast<float>(ns);
+        dr = w.x;
+        di = w.y;
+        orr = w.x * sr + w.y * si;  // conj(w) * avg
+        oi = w.x * si - w.y * sr;
+      }
+      dr = wsum(dr);
+      di = wsum(di);
+      orr = wsum(orr);
+      oi = wsum(oi);
+      if (hypotf(dr, di) < 1e-30f) {
+        float sr = 0.0f, si = 0.0f;
+        for (int a = lane; a < na; a += 64) {
+          sr += yc[act[a]].x;
+          si += yc[act[a]].y;
+        }
+        out = make_float2(wsum(sr), wsum(si));
+      } else {
+        const double dd = static_cast<double>(dr) * dr + static_cast<double>(di) * di;
+        const float nf = static_cast<float>(na);
+        out = make_float2(
+            static_cast<float>((static_cast<double>(orr) * dr + static_cast<double>(oi) * di) / dd) * nf,
+            static_cast<float>((static_cast<double>(oi) * dr - static_cast<double>(orr) * di) / dd) * nf);
+      }
+    }
+    if (M.cf) {
+      const float f = coherence(yc, act, na, lane);
+      out.x *= f;
+      out.y *= f;
+    }
+    if (lane == 0) store_out(M, pix, out, frame, acc);
   }
-  dr = wsum(dr);
-  di = wsum(di);
-  orr = wsum(orr);
-  oi = wsum(oi);
-  if (hypotf(dr, di) < 1e-30f) {
-    finish(das());
-    return;
-  }
-  // (out / den) * n_active
-  const double dd = static_cast<double>(dr) * dr + static_cast<double>(di) * di;
-  const float qr = static_cast<float>((static_cast<double>(orr) * dr + static_cast<double>(oi) * di) / dd);
-  const float qi = static_cast<float>((static_cast<double>(oi) * dr - static_cast<double>(orr) * di) / dd);
-  const float nf = static_cast<float>(na);
-  finish(make_float2(qr * nf, qi * nf));
+}
+
+template <int P>
+static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
+                          float* acc, const int* counts, const int* lists, hipStream_t st) {
+  const size_t smem = mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
+  const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
+  const int n_pix = M.n0 * M.n1;
+  const int per_cu = std::max(1, std::min(32, static_cast<int>(160 * 1024 / smem)));
+  const int grid = std::min(n_pix, 256 * per_cu);
+#define USX_MV(PR)                                                                       \
+  hipLaunchKernelGGL((mv_solve<P, PR>), dim3(grid), dim3(64), smem, st, M, Y, mask, frame, \
+                     acc, counts, lists, na_max)
+  if (M.precision == 1) USX_MV(1);
+  else if (M.precision == 2) USX_MV(2);
+  else USX_MV(0);
+#undef USX_MV
+}
+
+int mv_max_bucket_p(int n_ch, int subarray) {
+  const int lmax = std::min(std::max(subarray, 2), std::max(2, n_ch / 2));
+  return (lmax + 15) / 16 * 16;
 }
 
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
-               hipStream_t st) {
-  const int lmax = std::min(std::max(M.subarray, 2), std::max(2, M.n_ch / 2));
-  const int pmax = (lmax + 15) & ~15;
-  const size_t smem = mv_smem_bytes(M.n_ch, M.range_avg, pmax);
-  const dim3 grid(static_cast<unsigned>(M.n0) * M.n1), block(64);
-  if (M.precision == 1)
-    hipLaunchKernelGGL(mv_kernel<1>, grid, block, smem, st, M, Y, mask, frame, acc, pmax);
-  else if (M.precision == 2)
-    hipLaunchKernelGGL(mv_kernel<2>, grid, block, smem, st, M, Y, mask, frame, acc, pmax);
-  else
-    hipLaunchKernelGGL(mv_kernel<0>, grid, block, smem, st, M, Y, mask, frame, acc, pmax);
+               int* counts, int* lists, hipStream_t st) {
+  const int n_pix = M.n0 * M.n1;
+  hipLaunchKernelGGL(mv_zero_counts, dim3(1), dim3(64), 0, st, counts);
+  hipLaunchKernelGGL(mv_classify, dim3((n_pix + 255) / 256), dim3(256), 0, st, M, Y, mask, frame,
+                     acc, counts, lists);
+  const int pmax = mv_max_bucket_p(M.n_ch, M.subarray);
+  if (pmax >= 16) launch_bucket<16>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 32) launch_bucket<32>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 48) launch_bucket<48>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 64) launch_bucket<64>(M, Y, mask, frame, acc, counts, lists, st);
 }
 
 }  // namespace usxd
