@@ -2,25 +2,27 @@
 //
 // Reference: Beamformer::beamform, core/src/beamformer.cpp:297-549.
 //
-// Thread mapping. One thread owns one pixel. A workgroup is 4 wavefronts;
-// wavefront w of a workgroup covers 64 consecutive axis1 (depth) pixels of
-// lateral line i0 = 4*g + w, so the four waves of a workgroup read the same
-// IQ sample windows for a given (event, channel) and share them in L1. The
-// workgroup index is remapped so that each XCD (8 per MI355X, each with its
-// own L2) receives a contiguous range of tiles.
+// Thread mapping. A workgroup owns a tile of 64 consecutive axis1 (depth)
+// pixels of one lateral line, one pixel per lane; its 4 wavefronts split the
+// channel loop of those pixels (wave w takes channels lo + w, lo + w + 4, ...)
+// so four times as many wavefronts hide the latency of the sample gathers.
+// Tile indices are remapped so that each XCD (8 per MI355X, each with its own
+// L2) receives a contiguous range of tiles (neighbouring lines reuse IQ).
 //
-// Per thread: the transmit terms of each event slot (delay, illumination
-// weight times transmit phasor) are computed once and held in registers;
-// then the channel loop walks the union of the wavefront's active apertures,
-// computing the receive terms (aperture weight, delay, phasor) once per
-// (pixel, channel) and accumulating over the event slots:
+// Per pixel, the transmit terms of each event slot (delay, illumination
+// weight times transmit phasor) are computed once (split across the waves,
+// shared through LDS) and held in registers; then each receive channel's
+// terms (aperture weight, delay, phasor) are computed once per (pixel,
+// channel) and the event slots accumulated:
 //
 //   Y_ch = (a_ch e^{j phi_rx}) * sum_e v_e(idx) * (tw_e e^{j phi_tx,e})
 //
 // which is the reference's Y[ch] += v * (e^{j phi_tx} e^{j phi_rx}) * (a*tw)
 // (beamformer.cpp:470-478) with the receive factor taken out of the event
 // sum. The interpolation index, whose rounding decides which samples are
-// read, is evaluated in the reference's exact operation order.
+// read, is evaluated in the reference's exact operation order. DAS, DMAS and
+// coherence-factor sums are linear over channels, so each wave keeps partial
+// sums and the workgroup adds the four in a fixed order through LDS.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -43,7 +45,8 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Tile decode with XCD-aware remapping of the workgroup index.
+// Tile decode with XCD-aware remapping of the workgroup index: one tile is
+// 64 depths of one lateral line.
 __device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
@@ -52,11 +55,9 @@ __device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
   // Blocks beyond an uneven last XCD slice fall back to the identity map.
   if (nb % 8 != 0) t = b;
   const int nj = (A.g.n1 + 63) / 64;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int tg = t / nj, tj = t - tg * nj;
-  *i0 = tg * 4 + wave;
-  *j = tj * 64 + lane;
+  const int ti = t / nj, tj = t - ti * nj;
+  *i0 = ti;
+  *j = tj * 64 + static_cast<int>(threadIdx.x & 63);
 }
 
 struct TxSlot {
@@ -143,33 +144,79 @@ __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, boo
 
 }  // namespace
 
+// Transmit slots of the tile's pixels, computed by the four waves together
+// and returned in registers (slot s of this lane's pixel in sl[s]).
+template <int NE>
+__device__ __forceinline__ int tile_slots(const BfArgs& A, const Pt& p, bool valid, int ev_first,
+                                          int n_slots, int select, TxSlot (&sl)[NE],
+                                          TxSlot (*shared)[64]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int nsl = n_slots;
+  if (select) {
+    nsl = 1;
+    if (w == 0) {
+      float wb;
+      const int e = valid ? select_event(A, p, &wb) : -1;
+      shared[0][lane] = tx_slot(A, e < 0 ? A.ev_begin : e, (e >= 0 && wb > 0.0f) ? 1.0f : 0.0f, p);
+    }
+  } else {
+    for (int s = w; s < n_slots; s += 4)
+      shared[s][lane] = tx_slot(A, ev_first + s, valid ? tx_w(A, ev_first + s, p) : 0.0f, p);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < nsl) sl[s] = shared[s][lane];
+  return nsl;
+}
+
+// Sum over the event slots of one channel, then the receive rotation and
+// apodization: this lane's Y_ch (see the header).
+template <int NE, bool IQ>
+__device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
+                                                const TxSlot (&sl)[NE], int nsl, int ch, float a) {
+  const float trx = rx_time(A.ex[ch], p, A.c);
+  float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    if (s < nsl) {
+      const float idx = (sl[s].tau + trx - A.t0) * A.fs;
+      const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
+      const float2 v = sample<IQ>(data, base, A.n_s, idx);
+      acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
+      acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
+    }
+  }
+  float rr = a, ri = 0.0f;
+  if (A.fdem > 0.0f) {
+    float sn, cs;
+    sincosf(A.w0 * trx, &sn, &cs);
+    rr = a * cs;
+    ri = a * sn;
+  }
+  return make_float2(acc.x * rr - acc.y * ri, acc.x * ri + acc.y * rr);
+}
+
 // ---------------------------------------------------------------------------
-// Fused kernel: channel loop and DAS / DMAS combine (+ coherence factor) in
-// registers. NE = event slots per launch (compile-time, <= 16).
-//   select   : one slot per pixel, the pixel's best event, weight 1
+// Fused kernel: DAS / DMAS (+ coherence factor) straight to the frame.
+// NE = event slots per launch (compile-time, <= 16).
+//   select    : one slot per pixel, the pixel's best event, weight 1
 //   accumulate: add to the existing frame (DAS over event chunks; linear)
 // ---------------------------------------------------------------------------
 template <int NE, int COMB, bool CF, bool IQ>
 __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* data, float2* frame,
                                                        int ev_first, int n_slots, int select,
                                                        int accumulate) {
+  constexpr int NPART = 10;  // das(2) s(2) q(2) coh(2) inc na
+  __shared__ TxSlot shared_slots[NE][64];
+  __shared__ float part[4][NPART][64];
   int i0, j;
   tile_of(A, &i0, &j);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool valid = i0 < A.g.n0 && j < A.g.n1;
   const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
-
   TxSlot sl[NE];
-  int nsl = n_slots;
-  if (select) {
-    float wb;
-    const int e = valid ? select_event(A, p, &wb) : -1;
-    sl[0] = tx_slot(A, e < 0 ? A.ev_begin : e, (e >= 0 && wb > 0.0f) ? 1.0f : 0.0f, p);
-    nsl = 1;
-  } else {
-#pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n_slots) sl[s] = tx_slot(A, ev_first + s, valid ? tx_w(A, ev_first + s, p) : 0.0f, p);
-  }
+  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, select, sl, shared_slots);
 
   int lo, hi;
   aperture_range(A, p, valid, &lo, &hi);
@@ -178,31 +225,11 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
   float2 ds = das, dq = das, coh = das;
   float inc = 0.0f;
   int na = 0;
-  for (int ch = lo; ch <= hi; ++ch) {
-    const float ex = A.ex[ch];
-    const float a = valid ? aperture_weight(ex, p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
+  for (int ch = lo + w; ch <= hi; ch += 4) {
+    const float a = valid ? aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
     if (a <= 0.0f) continue;
     ++na;
-    const float trx = rx_time(ex, p, A.c);
-    float2 acc = make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (int s = 0; s < NE; ++s) {
-      if (s < nsl) {
-        const float idx = (sl[s].tau + trx - A.t0) * A.fs;
-        const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
-        const float2 v = sample<IQ>(data, base, A.n_s, idx);
-        acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
-        acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
-      }
-    }
-    float rr = a, ri = 0.0f;
-    if (A.fdem > 0.0f) {
-      float sn, cs;
-      sincosf(A.w0 * trx, &sn, &cs);
-      rr = a * cs;
-      ri = a * sn;
-    }
-    const float2 y = make_float2(acc.x * rr - acc.y * ri, acc.x * ri + acc.y * rr);
+    const float2 y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -213,32 +240,43 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
         const float vr = y.x / sq, vi = y.y / sq;
         ds.x += vr;
         ds.y += vi;
-        dq.x += vr * vr - vi * vi;
-        dq.y += vr * vi + vi * vr;
+        dq.x = fmaf(vr, vr, fmaf(-vi, vi, dq.x));
+        dq.y = fmaf(vr, vi, fmaf(vi, vr, dq.y));
       }
     }
     if (CF) {
       coh.x += y.x;
       coh.y += y.y;
       const float m = hypotf(y.x, y.y);
-      inc += m * m;
+      inc = fmaf(m, m, inc);
     }
   }
-  if (!valid) return;
+  const float mine[NPART] = {das.x, das.y, ds.x, ds.y, dq.x, dq.y, coh.x, coh.y, inc,
+                             static_cast<float>(na)};
+#pragma unroll
+  for (int q = 0; q < NPART; ++q) part[w][q][lane] = mine[q];
+  __syncthreads();
+  if (w != 0 || !valid) return;
+  float tot[NPART];
+#pragma unroll
+  for (int q = 0; q < NPART; ++q)
+    tot[q] = ((part[0][q][lane] + part[1][q][lane]) + part[2][q][lane]) + part[3][q][lane];
+  const int n_act = static_cast<int>(tot[9]);
   float2 out = make_float2(0.0f, 0.0f);
-  if (na > 0) {
+  if (n_act > 0) {
     if (COMB == 0) {
-      out = das;
+      out = make_float2(tot[0], tot[1]);
     } else {
       // ((s*s - q) * 0.5) / (0.5 * max(1, n - 1))  (beamformer.cpp:125-129)
-      const float ar = (ds.x * ds.x - ds.y * ds.y - dq.x) * 0.5f;
-      const float ai = (ds.x * ds.y + ds.y * ds.x - dq.y) * 0.5f;
-      const float nrm = 0.5f * smax(1.0f, static_cast<float>(na - 1));
+      const float sr = tot[2], si = tot[3];
+      const float ar = (sr * sr - si * si - tot[4]) * 0.5f;
+      const float ai = (sr * si + si * sr - tot[5]) * 0.5f;
+      const float nrm = 0.5f * smax(1.0f, static_cast<float>(n_act - 1));
       out = make_float2(ar / nrm, ai / nrm);
     }
     if (CF) {
-      const float mc = hypotf(coh.x, coh.y);
-      const float f = (inc > 0.0f) ? (mc * mc) / (static_cast<float>(na) * inc) : 0.0f;
+      const float mc = hypotf(tot[6], tot[7]);
+      const float f = (tot[8] > 0.0f) ? (mc * mc) / (static_cast<float>(n_act) * tot[8]) : 0.0f;
       out.x *= f;
       out.y *= f;
     }
@@ -253,78 +291,77 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
 }
 
 // ---------------------------------------------------------------------------
-// Channel-vector kernel: Y[pixel][ch] over event chunks, plus the active mask
-// (one bit per channel, a > 0). rect: MV's rectangular aperture
-// (beamformer.cpp:385). Y is accumulated across chunk launches; the first
-// chunk writes. Inactive channels are written as zero.
+// Channel-vector kernel: Y[pixel][ch] over event chunks, plus the active
+// mask (one bit per channel, a > 0). rect: MV's rectangular aperture
+// (beamformer.cpp:385). Only channels inside the tile's aperture union are
+// written (the mask marks which of them are active; consumers read no
+// other entry). Y is accumulated across chunk launches; the first writes.
+// Values are staged per 32-channel block in LDS so each pixel's row segment
+// is written as one contiguous 256-byte run.
 // ---------------------------------------------------------------------------
+constexpr int kMaskWordsMax = 16;  // up to 512 channels
+
 template <int NE, bool IQ>
 __global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* data, float2* Y,
                                                           uint32_t* mask, int ev_first,
                                                           int n_slots, int select, int first,
                                                           int rect) {
+  __shared__ TxSlot shared_slots[NE][64];
+  __shared__ float2 tile[64][33];
+  __shared__ uint32_t bits[64][kMaskWordsMax];
   int i0, j;
   tile_of(A, &i0, &j);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool valid = i0 < A.g.n0 && j < A.g.n1;
   const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
-  const size_t pix = static_cast<size_t>(i0) * A.g.n1 + j;
-
-  TxSlot sl[NE];
-  int nsl = n_slots;
-  if (select) {
-    float wb;
-    const int e = valid ? select_event(A, p, &wb) : -1;
-    sl[0] = tx_slot(A, e < 0 ? A.ev_begin : e, (e >= 0 && wb > 0.0f) ? 1.0f : 0.0f, p);
-    nsl = 1;
-  } else {
-#pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n_slots) sl[s] = tx_slot(A, ev_first + s, valid ? tx_w(A, ev_first + s, p) : 0.0f, p);
-  }
-
   const int nw = (A.n_ch + 31) / 32;
-  if (!valid) return;
-  float2* yp = Y + pix * A.n_ch;
-  uint32_t* mp = mask + pix * nw;
-  uint32_t bits = 0;
-  for (int ch = 0; ch < A.n_ch; ++ch) {
-    const float ex = A.ex[ch];
-    float a = aperture_weight(ex, p, A.f_number, A.rx_window, A.tukey_alpha);
-    if (rect) a = a > 0.0f ? 1.0f : 0.0f;
-    float2 y = make_float2(0.0f, 0.0f);
-    if (a > 0.0f) {
-      bits |= 1u << (ch & 31);
-      const float trx = rx_time(ex, p, A.c);
-      float2 acc = make_float2(0.0f, 0.0f);
-#pragma unroll
-      for (int s = 0; s < NE; ++s) {
-        if (s < nsl) {
-          const float idx = (sl[s].tau + trx - A.t0) * A.fs;
-          const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
-          const float2 v = sample<IQ>(data, base, A.n_s, idx);
-          acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
-          acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
+  for (int q = tid; q < 64 * kMaskWordsMax; q += 256) bits[q / kMaskWordsMax][q % kMaskWordsMax] = 0;
+  TxSlot sl[NE];
+  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, select, sl, shared_slots);
+
+  int lo, hi;
+  aperture_range(A, p, valid, &lo, &hi);
+  const size_t pix0 = static_cast<size_t>(i0) * A.g.n1 + (j - lane);
+  if (lo <= hi) {
+    for (int c0 = lo & ~31; c0 <= hi; c0 += 32) {
+      // each wave: channels c0 + w + 4k of this lane's pixel
+      for (int cc = w; cc < 32; cc += 4) {
+        const int ch = c0 + cc;
+        float2 y = make_float2(0.0f, 0.0f);
+        if (valid && ch >= lo && ch <= hi) {
+          float a = aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha);
+          if (rect) a = a > 0.0f ? 1.0f : 0.0f;
+          if (a > 0.0f) {
+            atomicOr(&bits[lane][ch >> 5], 1u << (ch & 31));
+            y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a);
+          }
+        }
+        tile[lane][cc] = y;
+      }
+      __syncthreads();
+      // write the block: 64 pixel rows x 32 channels, 4 lanes per 32-byte piece
+      const int ncols = min(32, A.n_ch - c0);
+      for (int q = tid; q < 64 * 32; q += 256) {
+        const int r = q >> 5, cc = q & 31;
+        const int jr = j - lane + r;
+        if (cc < ncols && jr < A.g.n1 && i0 < A.g.n0) {
+          float2* dst = Y + (pix0 + r) * A.n_ch + c0 + cc;
+          float2 v = tile[r][cc];
+          if (!first) {
+            const float2 prev = *dst;
+            v.x += prev.x;
+            v.y += prev.y;
+          }
+          *dst = v;
         }
       }
-      float rr = a, ri = 0.0f;
-      if (A.fdem > 0.0f) {
-        float sn, cs;
-        sincosf(A.w0 * trx, &sn, &cs);
-        rr = a * cs;
-        ri = a * sn;
-      }
-      y = make_float2(acc.x * rr - acc.y * ri, acc.x * ri + acc.y * rr);
+      __syncthreads();
     }
-    if (!first) {
-      const float2 prev = yp[ch];
-      y.x += prev.x;
-      y.y += prev.y;
-    }
-    yp[ch] = y;
-    if ((ch & 31) == 31 || ch == A.n_ch - 1) {
-      mp[ch >> 5] = bits;
-      bits = 0;
-    }
+  }
+  __syncthreads();
+  if (valid && w == 0) {
+    uint32_t* mp = mask + (pix0 + lane) * nw;
+    for (int q = 0; q < nw; ++q) mp[q] = bits[lane][q];
   }
 }
 
@@ -415,9 +452,7 @@ __global__ void fill_zero_kernel(float* p, size_t n) {
 // Launchers
 // ---------------------------------------------------------------------------
 
-static int tile_blocks(const BfArgs& A) {
-  return ((A.g.n0 + 3) / 4) * ((A.g.n1 + 63) / 64);
-}
+static int tile_blocks(const BfArgs& A) { return A.g.n0 * ((A.g.n1 + 63) / 64); }
 
 template <int NE, bool IQ>
 static void launch_fused_ne(const BfArgs& A, const void* data, float2* frame, int ev_first,

@@ -142,6 +142,7 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (!b) fail(c, USX_ERR_INVALID, "usx_beamform: null descriptor");
   if (b->n_events <= 0) fail(c, USX_ERR_INVALID, "ChannelData has no transmit events");
   if (b->n_channels <= 0) fail(c, USX_ERR_INVALID, "Probe.n_elements must be > 0");
+  if (b->n_channels > 512) fail(c, USX_ERR_INVALID, "beamform: at most 512 channels are supported");
   if (b->n_samples <= 0) fail(c, USX_ERR_INVALID, "Acquisition.n_samples must be > 0");
   if (b->event_begin < 0 || b->event_end < b->event_begin || b->event_end > b->n_events)
     fail(c, USX_ERR_INVALID, "beamform: event range [%d, %d) outside [0, %d]", b->event_begin,

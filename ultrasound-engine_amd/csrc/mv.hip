@@ -225,7 +225,15 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       if (k < 0 || k >= M.n1) continue;
       if (d == 0) crow = nrow;
       const float2* src = Y + (static_cast<size_t>(i0) * M.n1 + k) * n_ch;
-      for (int a = lane; a < na; a += 64) rows[nrow * na + a] = src[act[a]];
+      // A channel inactive at the neighbouring depth contributes 0 there
+      // (its Y was never accumulated, beamformer.cpp:469); the channel
+      // kernel does not write such entries, so test the neighbour's mask.
+      const uint32_t* mk = mask + (static_cast<size_t>(i0) * M.n1 + k) * nw;
+      for (int a = lane; a < na; a += 64) {
+        const int ch = act[a];
+        const bool on = d == 0 || ((mk[ch >> 5] >> (ch & 31)) & 1u);
+        rows[nrow * na + a] = on ? src[ch] : make_float2(0.0f, 0.0f);
+      }
       ++nrow;
     }
     __syncthreads();
