@@ -153,7 +153,8 @@ typedef struct {
   float f_number;
   int rx_window;           /* 0 Rect 1 Hann 2 Hamming 3 Tukey 4 Blackman */
   float tukey_alpha;
-  int combiner;            /* 0 DAS 1 DMAS 2 MV */
+  int combiner;            /* 0 DAS 1 DMAS 2 MV; 3 learned apodization (no reference
+                              counterpart: needs 128 channels and a network, see usx_apod_*) */
   int compounding;         /* 0 Coherent 1 Incoherent 2 Select */
   int transmit_weighting;
   int coherence_factor;
@@ -184,6 +185,69 @@ typedef struct {
 usx_status usx_process_frame(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
                              const float* rf, usx_c32* frame, int memory,
                              usx_stage_times* times);
+
+/* ---- minimum-variance channel weights (no reference counterpart) ------- */
+
+/* The minimum-variance frame (b->combiner must be 2, coherent or select
+ * compounding) together with each pixel's effective channel weights g, such
+ * that, before the coherence factor, out = sum_ch g[ch] * Y[ch] over the
+ * rectangular-aperture channel vector Y (beamformer.cpp:202-213 rewritten
+ * per channel: g_c = n / (n_sub sum w) * sum over the subarrays covering c
+ * of conj(w_a)); inactive channels get 0 and DAS fallbacks get 1.
+ * weights: n_axis0 * n_axis1 * n_channels complex; status (may be NULL):
+ * n_axis0 * n_axis1 ints, 0 solved, 1 fewer than 4 active channels,
+ * 2 solve failed (both DAS), 3 no active channel. */
+usx_status usx_mv_weights(usx_ctx* ctx, const usx_bf_desc* b, const void* data, usx_c32* frame,
+                          usx_c32* weights, int* status, int memory);
+
+/* ---- learned apodization (no reference counterpart) ---------------------
+ *
+ * A 256 -> 256 -> 256 -> 256 -> 256 MLP (tanh-form GELU after the first three
+ * layers; bf16 weights and activations on MFMA, FP32 accumulation) maps a
+ * pixel's rectangular-aperture channel vector (128 channels, normalised to
+ * unit mean power and the phase of its sum) to channel weights
+ * 1 + o_c; combiner 3 forms out = sum_c (1 + o_c) Y_c over the active
+ * channels (DAS below 4 active channels, as minimum variance does). It is
+ * trained on the minimum-variance weights of usx_mv_weights. Parameters are
+ * FP32, USX_APOD_PARAMS floats: W_0..W_3 (each [in][out], 256 x 256), then
+ * b_0..b_3 (256 each). Features are (re, im) interleaved per channel. */
+#define USX_APOD_CHANNELS 128
+#define USX_APOD_FEATURES 256
+#define USX_APOD_PARAMS (4 * 256 * 256 + 4 * 256)
+
+/* Load parameters (host, USX_APOD_PARAMS floats); resets the optimizer. */
+usx_status usx_apod_set_params(usx_ctx* ctx, const float* params);
+usx_status usx_apod_get_params(usx_ctx* ctx, float* params);
+
+/* Training set held on the device: `capacity` rows, emptied. */
+usx_status usx_apod_dataset_reserve(usx_ctx* ctx, size_t capacity);
+usx_status usx_apod_dataset_size(usx_ctx* ctx, size_t* n);
+/* Append the listed pixels of one frame: run the minimum-variance path on
+ * the data (RF through d when d is not NULL, else IQ as b describes it;
+ * b->combiner must be 2), then store each listed pixel's network input,
+ * weights, active mask and sample weight (1/n_active; 0 for pixels that
+ * fell back to DAS). pixels: host, n_pixels indices into the frame
+ * (axis0 * n_axis1 + axis1). frame (host, may be NULL) receives the MV frame. */
+usx_status usx_apod_dataset_append(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
+                                   const void* data, int memory, const int* pixels, int n_pixels,
+                                   usx_c32* frame);
+
+typedef struct {
+  int n_steps;
+  int batch;               /* multiple of 256 */
+  float lr, beta1, beta2, eps, weight_decay;
+  unsigned long long seed; /* minibatch rows are drawn by hash(seed, step, b) */
+} usx_apod_train_desc;
+/* n_steps Adam steps on minibatches of the device training set. losses
+ * (host, may be NULL) receives each step's loss. */
+usx_status usx_apod_train(usx_ctx* ctx, const usx_apod_train_desc* t, float* losses);
+
+/* Loss and gradient of one batch given on the host, with the current
+ * parameters (no update). x: n * 256 bf16 bit patterns; g: n * 256 floats;
+ * mask: n * 4 words; sw: n sample weights; grads: USX_APOD_PARAMS floats.
+ * n <= 4096. */
+usx_status usx_apod_gradient(usx_ctx* ctx, int n, const uint16_t* x, const float* g,
+                             const uint32_t* mask, const float* sw, float* loss, float* grads);
 
 #ifdef __cplusplus
 }

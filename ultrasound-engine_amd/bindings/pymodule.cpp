@@ -13,6 +13,8 @@
 
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <algorithm>
 
 #include "../core/src/gpu.hpp"
 #include "usx/beamformer.hpp"
@@ -128,6 +130,129 @@ class DeviceRunner {
   bool is_rf() const { return is_rf_; }
   int iq_samples() const { return bplan_.desc.n_samples; }
 
+  // ---- learned apodization (no reference counterpart) ----------------------
+  using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+  void set_apodization(farr p) {
+    if (p.size() != USX_APOD_PARAMS)
+      throw std::invalid_argument("set_apodization: expected " + std::to_string(USX_APOD_PARAMS) +
+                                  " parameters");
+    usx_ctx* c = ctx_.get();
+    check(c, usx_apod_set_params(c, p.data()), "apod_set_params");
+  }
+  py::array_t<float> get_apodization() {
+    py::array_t<float> p(USX_APOD_PARAMS);
+    usx_ctx* c = ctx_.get();
+    check(c, usx_apod_get_params(c, p.mutable_data()), "apod_get_params");
+    return p;
+  }
+  // The descriptor with the minimum-variance combiner (the labels' source).
+  usx_bf_desc mv_desc() const {
+    usx_bf_desc b = bplan_.desc;
+    b.combiner = 2;
+    return b;
+  }
+  // Host ChannelData (the template's shape) -> the IQ the plan beamforms.
+  std::vector<usx_c32> host_iq(const ChannelData& data) {
+    usx_ctx* c = ctx_.get();
+    const auto& b = bplan_.desc;
+    std::vector<usx_c32> iq(static_cast<std::size_t>(b.n_events) * b.n_channels * b.n_samples);
+    if (is_rf_) {
+      if (data.rf.size() != static_cast<std::size_t>(dplan_.desc.n_traces) * dplan_.desc.n_in)
+        throw std::invalid_argument("channel data does not match the runner's template");
+      check(c, usx_demodulate(c, &dplan_.desc, data.rf.data(), iq.data(), USX_HOST), "demodulate");
+    } else {
+      if (data.iq.size() != iq.size())
+        throw std::invalid_argument("channel data does not match the runner's template");
+      std::memcpy(iq.data(), data.iq.data(), iq.size() * sizeof(usx_c32));
+    }
+    return iq;
+  }
+  py::tuple mv_weights(const ChannelData& data) {
+    const auto& b = bplan_.desc;
+    const std::vector<usx_c32> iq = host_iq(data);
+    py::array_t<std::complex<float>> frame({b.n_axis0, b.n_axis1});
+    py::array_t<std::complex<float>> w({b.n_axis0, b.n_axis1, b.n_channels});
+    py::array_t<int> st({b.n_axis0, b.n_axis1});
+    const usx_bf_desc d = mv_desc();
+    usx_ctx* c = ctx_.get();
+    {
+      py::gil_scoped_release release;
+      check(c, usx_mv_weights(c, &d, iq.data(), reinterpret_cast<usx_c32*>(frame.mutable_data()),
+                              reinterpret_cast<usx_c32*>(w.mutable_data()), st.mutable_data(), USX_HOST),
+            "mv_weights");
+    }
+    return py::make_tuple(frame, w, st);
+  }
+  void dataset_reserve(std::size_t n) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_apod_dataset_reserve(c, n), "apod_dataset_reserve");
+  }
+  std::size_t dataset_size() {
+    usx_ctx* c = ctx_.get();
+    std::size_t n = 0;
+    check(c, usx_apod_dataset_size(c, &n), "apod_dataset_size");
+    return n;
+  }
+  py::array_t<std::complex<float>> dataset_append(
+      const ChannelData& data, py::array_t<int, py::array::c_style | py::array::forcecast> pixels) {
+    const auto& b = bplan_.desc;
+    py::array_t<std::complex<float>> frame({b.n_axis0, b.n_axis1});
+    const usx_bf_desc d = mv_desc();
+    usx_ctx* c = ctx_.get();
+    const void* src;
+    if (is_rf_) {
+      if (data.rf.size() != static_cast<std::size_t>(dplan_.desc.n_traces) * dplan_.desc.n_in)
+        throw std::invalid_argument("channel data does not match the runner's template");
+      src = data.rf.data();
+    } else {
+      if (data.iq.size() != static_cast<std::size_t>(b.n_events) * b.n_channels * b.n_samples)
+        throw std::invalid_argument("channel data does not match the runner's template");
+      src = data.iq.data();
+    }
+    {
+      py::gil_scoped_release release;
+      check(c, usx_apod_dataset_append(c, is_rf_ ? &dplan_.desc : nullptr, &d, src, USX_HOST,
+                                       pixels.data(), static_cast<int>(pixels.size()),
+                                       reinterpret_cast<usx_c32*>(frame.mutable_data())),
+            "apod_dataset_append");
+    }
+    return frame;
+  }
+  py::array_t<float> train(int n_steps, int batch, float lr, float beta1, float beta2, float eps,
+                           float weight_decay, unsigned long long seed) {
+    usx_apod_train_desc t{};
+    t.n_steps = n_steps;
+    t.batch = batch;
+    t.lr = lr;
+    t.beta1 = beta1;
+    t.beta2 = beta2;
+    t.eps = eps;
+    t.weight_decay = weight_decay;
+    t.seed = seed;
+    py::array_t<float> losses(std::max(n_steps, 0));
+    usx_ctx* c = ctx_.get();
+    {
+      py::gil_scoped_release release;
+      check(c, usx_apod_train(c, &t, losses.mutable_data()), "apod_train");
+    }
+    return losses;
+  }
+  py::tuple gradient(py::array_t<uint16_t, py::array::c_style | py::array::forcecast> x, farr g,
+                     py::array_t<uint32_t, py::array::c_style | py::array::forcecast> mask,
+                     farr sw) {
+    const py::ssize_t n = sw.size();
+    if (x.size() != n * USX_APOD_FEATURES || g.size() != n * USX_APOD_FEATURES || mask.size() != n * 4)
+      throw std::invalid_argument("gradient: x, g (n x 256), mask (n x 4) and sw (n) disagree");
+    float loss = 0.0f;
+    py::array_t<float> grads(USX_APOD_PARAMS);
+    usx_ctx* c = ctx_.get();
+    check(c, usx_apod_gradient(c, static_cast<int>(n), x.data(), g.data(), mask.data(), sw.data(),
+                               &loss, grads.mutable_data()),
+          "apod_gradient");
+    return py::make_tuple(loss, grads);
+  }
+
  private:
   ScanGrid grid_;
   PipelineConfig cfg_;
@@ -158,7 +283,8 @@ PYBIND11_MODULE(_usx, m) {
   py::enum_<Combiner>(m, "Combiner")
       .value("DAS", Combiner::DAS)
       .value("DMAS", Combiner::DMAS)
-      .value("MV", Combiner::MV);
+      .value("MV", Combiner::MV)
+      .value("Learned", Combiner::Learned);
   py::enum_<Compounding>(m, "Compounding")
       .value("COHERENT", Compounding::Coherent)
       .value("INCOHERENT", Compounding::Incoherent)
@@ -608,6 +734,13 @@ PYBIND11_MODULE(_usx, m) {
           },
           py::arg("frame"), py::arg("medium") = Medium{})
       .def("reset", &ImagingPipeline::reset)
+      .def(
+          "set_apodization",
+          [](ImagingPipeline& p, py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+            p.set_apodization(std::vector<float>(a.data(), a.data() + a.size()));
+          },
+          py::arg("params"))
+      .def_property_readonly("has_apodization", &ImagingPipeline::has_apodization)
       .def_property_readonly("grid", &ImagingPipeline::grid)
       .def_property_readonly("stats", &ImagingPipeline::stats)
       .def_property_readonly("frame_count", &ImagingPipeline::frame_count)
@@ -627,7 +760,20 @@ PYBIND11_MODULE(_usx, m) {
       .def("profile_report", &DeviceRunner::profile_report)
       .def_property_readonly("input_shape", &DeviceRunner::input_shape)
       .def_property_readonly("is_rf", &DeviceRunner::is_rf)
-      .def_property_readonly("iq_samples", &DeviceRunner::iq_samples);
+      .def_property_readonly("iq_samples", &DeviceRunner::iq_samples)
+      .def("set_apodization", &DeviceRunner::set_apodization, py::arg("params"))
+      .def("get_apodization", &DeviceRunner::get_apodization)
+      .def("mv_weights", &DeviceRunner::mv_weights, py::arg("channel_data"))
+      .def("dataset_reserve", &DeviceRunner::dataset_reserve, py::arg("capacity"))
+      .def_property_readonly("dataset_size", &DeviceRunner::dataset_size)
+      .def("dataset_append", &DeviceRunner::dataset_append, py::arg("channel_data"),
+           py::arg("pixels"))
+      .def("train", &DeviceRunner::train, py::arg("n_steps"), py::arg("batch") = 4096,
+           py::arg("lr") = 1e-3f, py::arg("beta1") = 0.9f, py::arg("beta2") = 0.999f,
+           py::arg("eps") = 1e-8f, py::arg("weight_decay") = 0.0f, py::arg("seed") = 0ull)
+      .def("gradient", &DeviceRunner::gradient, py::arg("x"), py::arg("g"), py::arg("mask"),
+           py::arg("sw"));
+  m.attr("APOD_PARAMS") = USX_APOD_PARAMS;
 
   m.def("build_info", []() { return std::string(usx_build_info()); });
   m.attr("__version__") = "0.1.0";

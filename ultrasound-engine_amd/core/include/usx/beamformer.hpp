@@ -10,7 +10,9 @@
 
 namespace usx {
 
-enum class Combiner : std::uint8_t { DAS = 0, DMAS = 1, MV = 2 };
+// Learned: the learned-apodization network (no reference counterpart), with
+// minimum variance as its fallback when no network is loaded.
+enum class Combiner : std::uint8_t { DAS = 0, DMAS = 1, MV = 2, Learned = 3 };
 enum class Compounding : std::uint8_t { Coherent = 0, Incoherent = 1, Select = 2 };
 
 // Covariance input precision of the MV combine (no reference counterpart;

@@ -56,6 +56,12 @@ class ImagingPipeline {
   std::int64_t frame_count() const { return frame_count_; }
   void reset();
 
+  // Learned apodization (no reference counterpart): the network's FP32
+  // parameters (USX_APOD_PARAMS floats, include/usx_gpu.h). With
+  // Combiner::Learned and no network loaded, frames are minimum variance.
+  void set_apodization(const std::vector<float>& params);
+  bool has_apodization() const { return has_apod_; }
+
  private:
   ScanGrid grid_;
   PipelineConfig config_;
@@ -63,6 +69,7 @@ class ImagingPipeline {
   real reference_ = 0.0f;
   std::int64_t frame_count_ = 0;
   std::unique_ptr<GpuContext> gpu_;
+  bool has_apod_ = false;
 };
 
 }  // namespace usx

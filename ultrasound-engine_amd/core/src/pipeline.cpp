@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <stdexcept>
+#include <string>
 
 #include "gpu.hpp"
 
@@ -34,12 +36,23 @@ void ImagingPipeline::reset() {
   stats_ = PipelineStats{};
 }
 
+void ImagingPipeline::set_apodization(const std::vector<float>& params) {
+  if (params.size() != static_cast<std::size_t>(USX_APOD_PARAMS))
+    throw std::invalid_argument("set_apodization: expected " + std::to_string(USX_APOD_PARAMS) +
+                                " parameters");
+  usx_ctx* ctx = gpu_->get();
+  check(ctx, usx_apod_set_params(ctx, params.data()), "apod_set_params");
+  has_apod_ = true;
+}
+
 Frame ImagingPipeline::process_frame(const ChannelData& data) {
   const auto t_start = Clock::now();
   stats_ = PipelineStats{};
   data.validate();
   config_.beamformer.validate();
   usx_ctx* ctx = gpu_->get();
+  BeamformerConfig bcfg = config_.beamformer;
+  if (bcfg.combiner == Combiner::Learned && !has_apod_) bcfg.combiner = Combiner::MV;
 
   Frame frame;
   frame.grid = grid_;
@@ -50,7 +63,7 @@ Frame ImagingPipeline::process_frame(const ChannelData& data) {
     frame.demodulation_frequency = dp.fdem;
     frame.allocate();
     const BeamformPlan bp = make_bf_plan(data, dp.n_out, dp.fs_out, true, dp.fdem, grid_,
-                                         config_.beamformer, 0, n_ev);
+                                         bcfg, 0, n_ev);
     usx_stage_times t{};
     check(ctx, usx_process_frame(ctx, &dp.desc, &bp.desc, data.rf.data(),
                                  reinterpret_cast<usx_c32*>(frame.data.data()), USX_HOST, &t),
@@ -63,8 +76,7 @@ Frame ImagingPipeline::process_frame(const ChannelData& data) {
     frame.demodulation_frequency = data.demodulation_frequency;
     frame.allocate();
     const BeamformPlan bp = make_bf_plan(data, data.n_samples(), data.acq.sampling_frequency, true,
-                                         data.demodulation_frequency, grid_, config_.beamformer,
-                                         0, n_ev);
+                                         data.demodulation_frequency, grid_, bcfg, 0, n_ev);
     const auto t = Clock::now();
     check(ctx, usx_beamform(ctx, &bp.desc, data.iq.data(),
                             reinterpret_cast<usx_c32*>(frame.data.data()), USX_HOST),

@@ -1,0 +1,673 @@
+// apod.hip -- learned apodization on gfx950: a 256 -> 256 -> 256 -> 256 -> 256
+// MLP (tanh-form GELU after the first three layers) that maps a pixel's delayed
+// channel vector (128 complex samples, rectangular aperture) to 128 complex
+// channel weights. BF16 operands on MFMA (v_mfma_f32_16x16x32_bf16), FP32
+// accumulation; biases, pre-activations and the output in FP32.
+//
+// This has no counterpart in the reference. The network is trained on the
+// effective channel weights of the minimum-variance beamformer (mv.hip),
+// and the frame it produces is out = sum_c (1 + o_c) Y_c over the active
+// channels: a zero network is DAS with the rectangular aperture.
+//
+// Layout. Features are interleaved (re, im) per channel: f = 2 ch + {0, 1}.
+// Every GEMM runs transposed, out^T = W^T in^T, so samples are the MFMA N
+// dimension (lane % 16) and features the M and K dimensions. A 16x16 output
+// tile leaves lane (g = lane / 16, r = lane % 16) holding features
+// 16 m + 4 g + i (i = 0..3) of sample r; two neighbouring tiles (m = 2 kb,
+// 2 kb + 1) are then exactly the B operand of K block kb of the next layer
+// with the K order
+//   f(kb, g, j) = 32 kb + 4 g + j        (j < 4)
+//               = 32 kb + 16 + 4 g + j-4 (j >= 4),
+// so activations never leave registers between layers. The weights are
+// stored pre-swizzled in that K order: fragment (m, kb) is 64 lanes x 8
+// bf16 = 1 KB contiguous, lane (g, r) element j = W[f(kb, g, j)][16 m + r]
+// (forward) or W[16 m + r][f(kb, g, j)] (backward, dIn = W dOut). The sum
+// over K does not depend on which K index the hardware pairs with which
+// (g, j), only on A and B using the same map, which they do by construction.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace usxd {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+constexpr int kF = kApodFeatures;    // 256
+constexpr int kL = kApodLayers;      // 4
+constexpr int kMT = kF / 16;         // 16 output tiles
+constexpr int kKB = kF / 32;         // 8 K blocks
+constexpr int kLayerBf16 = kF * kF;  // swizzled bf16 per layer
+constexpr int kNT = 1;               // 16-sample tiles per wavefront
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ uint32_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ uint32_t pack2(float a, float b) { return f2bf(a) | (f2bf(b) << 16); }
+
+__device__ __forceinline__ bf8 as_bf8(u4 v) { return __builtin_bit_cast(bf8, v); }
+
+// GELU in its tanh form, 0.5 z (1 + tanh(k (z + 0.044715 z^3))), k = sqrt(2 / pi),
+// and its derivative; tanh(u) = 1 - 2 / (1 + e^{2u}) (saturates correctly at
+// both ends: e^{2u} = inf gives 1, e^{2u} = 0 gives -1).
+constexpr float kGeluK = 0.7978845608028654f;
+constexpr float kGeluC = 0.044715f;
+__device__ __forceinline__ float tanh_fast(float u) {
+  const float e = __expf(2.0f * u);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ float gelu(float z) {
+  const float t = tanh_fast(kGeluK * (z + kGeluC * z * z * z));
+  return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float t = tanh_fast(kGeluK * (z + kGeluC * z * z * z));
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluK * (1.0f + 3.0f * kGeluC * z * z);
+}
+
+// Swizzled position of W[in][out] in the forward (A = W^T) and backward
+// (A = W) fragment layouts.
+__host__ __device__ __forceinline__ int frag_pos(int row, int k) {
+  const int m = row >> 4, r = row & 15, kb = k >> 5, q = k & 31;
+  const int g = (q & 15) >> 2, j = (q & 3) + ((q >> 4) << 2);
+  return ((m * kKB + kb) * 64 + g * 16 + r) * 8 + j;
+}
+
+// One layer on a wavefront's kNT sample tiles: acc[m][t] = bias + A_m . act[.][t]
+// with A fragments from `w` (LDS or global, 16 bytes per lane per fragment).
+// The fragments of tile m + 1 are loaded while tile m's MFMAs run; the
+// scheduling barrier keeps the compiler from hoisting every load of the
+// layer (128 fragments) to the top, which would need 512 VGPRs.
+template <bool kGlobal>
+__device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (&act)[kKB][kNT],
+                                      f4 (&acc)[kMT][kNT], int lane) {
+  const int g = lane >> 4;
+  u4 cur[kKB], nxt[kKB];
+  auto load = [&](int m, u4 (&dst)[kKB]) {
+    const u4* wm = w + m * kKB * 64;
+    // Global fragments: an opaque per-tile uniform base, so the compiler
+    // addresses each load as SGPR base + lane offset instead of hoisting
+    // 128 per-lane 64-bit addresses out of the layer loop.
+    if (kGlobal) asm volatile("" : "+s"(wm));
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb) dst[kb] = wm[kb * 64 + lane];
+  };
+  load(0, cur);
+#pragma unroll
+  for (int m = 0; m < kMT; ++m) {
+    if (m + 1 < kMT) load(m + 1, nxt);
+    f4 b = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (bias) {
+      const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * m + 4 * g);
+      b = f4{bb.x, bb.y, bb.z, bb.w};
+    }
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) acc[m][t] = b;
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb)
+#pragma unroll
+      for (int t = 0; t < kNT; ++t)
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(cur[kb]), as_bf8(act[kb][t]),
+                                                            acc[m][t], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb) cur[kb] = nxt[kb];
+  }
+}
+
+// Network input of one pixel from its channel vector: the active channels
+// (mask bit set) rotated by conj(sum y) / |sum y| and scaled to unit mean
+// power, in BF16. Lane (g, r) of tile t holds pixel r's features for K
+// blocks kb = 0..7: channels 16 kb + 2 g + {0, 1} and 16 kb + 8 + 2 g + {0, 1}.
+// Inactive channels and invalid pixels give 0. Returns the active count.
+__device__ __forceinline__ int load_features(const float2* Y, const uint32_t* mask, int n_ch,
+                                             long pix, bool valid, int lane, u4 (&x)[kKB]) {
+  const int g = lane >> 4;
+  const int nw = (n_ch + 31) / 32;
+  float4 v[kKB][2];
+  uint32_t mw[4] = {0u, 0u, 0u, 0u};
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mw[q] = mask[pix * nw + q];
+  }
+  float sr = 0.0f, si = 0.0f, e = 0.0f;
+  int na = 0;
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ch = 16 * kb + 8 * h + 2 * g;
+      const uint32_t bits = mw[ch >> 5] >> (ch & 31);
+      float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (valid && (bits & 3u)) y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
+      if (!(bits & 1u)) y.x = y.y = 0.0f;
+      if (!(bits & 2u)) y.z = y.w = 0.0f;
+      v[kb][h] = y;
+      sr += y.x + y.z;
+      si += y.y + y.w;
+      e += y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w;
+      na += __popc(bits & 3u);
+    }
+  }
+#pragma unroll
+  for (int o = 16; o < 64; o <<= 1) {
+    sr += __shfl_xor(sr, o, 64);
+    si += __shfl_xor(si, o, 64);
+    e += __shfl_xor(e, o, 64);
+    na += __shfl_xor(na, o, 64);
+  }
+  float cr = 1.0f, ci = 0.0f, s = 0.0f;
+  if (e > 0.0f && na > 0) {
+    const float mag = sqrtf(sr * sr + si * si);
+    if (mag > 0.0f) {
+      cr = sr / mag;
+      ci = -si / mag;
+    }
+    s = 1.0f / sqrtf(e / static_cast<float>(na));
+  }
+  cr *= s;
+  ci *= s;
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+    const float4 a = v[kb][0], b = v[kb][1];
+    x[kb] = u4{pack2(a.x * cr - a.y * ci, a.x * ci + a.y * cr),
+               pack2(a.z * cr - a.w * ci, a.z * ci + a.w * cr),
+               pack2(b.x * cr - b.y * ci, b.x * ci + b.y * cr),
+               pack2(b.z * cr - b.w * ci, b.z * ci + b.w * cr)};
+  }
+  return na;
+}
+
+// The next layer's B operand from output tiles 2 kb and 2 kb + 1, through GELU.
+__device__ __forceinline__ void activate(const f4 (&acc)[kMT][kNT], u4 (&act)[kKB][kNT]) {
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb)
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      const f4 a = acc[2 * kb][t], b = acc[2 * kb + 1][t];
+      act[kb][t] = u4{pack2(gelu(a[0]), gelu(a[1])), pack2(gelu(a[2]), gelu(a[3])),
+                      pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
+      // bound the interleaving of the erff evaluations (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Inference: one workgroup = 8 wavefronts x 32 pixels; each layer's 128 KB
+// of swizzled weights is staged in LDS and shared by the 8 wavefronts.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
+                                                         const float2* __restrict__ Y,
+                                                         const uint32_t* __restrict__ mask,
+                                                         float2* __restrict__ frame, float* acc_out,
+                                                         int cf, int incoherent) {
+  extern __shared__ u4 wl[];  // kLayerBf16 bf16
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const long base = (static_cast<long>(blockIdx.x) * 8 + wv) * (16 * kNT);
+  u4 act[kKB][kNT];
+  int na[kNT];
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const long pix = base + 16 * t + r;
+    u4 x[kKB];
+    na[t] = load_features(Y, mask, n_ch, pix, pix < n_pix, lane, x);
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb) act[kb][t] = x[kb];
+  }
+  f4 acc[kMT][kNT];
+  const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
+  for (int l = 0; l < kL; ++l) {
+    __syncthreads();
+    for (int q = tid; q < kLayerBf16 / 8; q += 512) wl[q] = wsrc[l * (kLayerBf16 / 8) + q];
+    __syncthreads();
+    layer<false>(wl, N.bias + l * kF, act, acc, lane);
+    if (l + 1 < kL) activate(acc, act);
+  }
+  // out = sum over active channels of (1 + o_c) y_c; fewer than 4 active
+  // channels -> the DAS sum, as minimum variance does.
+  const int nw = (n_ch + 31) / 32;
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const long pix = base + 16 * t + r;
+    const bool valid = pix < n_pix;
+    uint32_t mw[4] = {0u, 0u, 0u, 0u};
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mw[q] = mask[pix * nw + q];
+    }
+    float lr = 0.0f, li = 0.0f, dr = 0.0f, di = 0.0f, inc = 0.0f;
+#pragma unroll
+    for (int m = 0; m < kMT; ++m) {
+      const int ch = 8 * m + 2 * g;
+      const uint32_t bits = mw[ch >> 5] >> (ch & 31);
+      if (!(bits & 3u)) continue;
+      float4 y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
+      if (!(bits & 1u)) y.x = y.y = 0.0f;
+      if (!(bits & 2u)) y.z = y.w = 0.0f;
+      const f4 o = acc[m][t];
+      const float g0r = (bits & 1u) ? 1.0f + o[0] : 0.0f, g0i = (bits & 1u) ? o[1] : 0.0f;
+      const float g1r = (bits & 2u) ? 1.0f + o[2] : 0.0f, g1i = (bits & 2u) ? o[3] : 0.0f;
+      lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
+      li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
+      dr += y.x + y.z;
+      di += y.y + y.w;
+      inc += y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w;
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      lr += __shfl_xor(lr, o, 64);
+      li += __shfl_xor(li, o, 64);
+      dr += __shfl_xor(dr, o, 64);
+      di += __shfl_xor(di, o, 64);
+      inc += __shfl_xor(inc, o, 64);
+    }
+    if (!valid || g != 0) continue;
+    float2 out = na[t] >= 4 ? make_float2(lr, li) : make_float2(dr, di);
+    if (cf) {
+      const float mc2 = dr * dr + di * di;
+      const float f = (inc > 0.0f && na[t] > 0) ? mc2 / (static_cast<float>(na[t]) * inc) : 0.0f;
+      out.x *= f;
+      out.y *= f;
+    }
+    if (incoherent) {
+      acc_out[pix] += out.x * out.x + out.y * out.y;
+    } else {
+      frame[pix] = out;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training set rows from a frame's channel vectors and minimum-variance
+// weights: X (bf16 features, the inference kernel's input), G (weights as
+// interleaved floats), mask words and the sample weight 1 / n_active (0 when
+// minimum variance fell back to DAS or the pixel has < 4 active channels).
+// One wavefront = 32 listed pixels, the inference kernel's lane mapping.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void apod_gather_kernel(int n_list, const int* __restrict__ pixels,
+                                                         int n_ch, const float2* __restrict__ Y,
+                                                         const uint32_t* __restrict__ mask,
+                                                         const float2* __restrict__ weights,
+                                                         const int* __restrict__ status,
+                                                         ApodData D, long dst) {
+  const int lane = threadIdx.x, g = lane >> 4, r = lane & 15;
+  const int nw = (n_ch + 31) / 32;
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const int i = blockIdx.x * (16 * kNT) + 16 * t + r;
+    const bool valid = i < n_list;
+    const long pix = valid ? pixels[i] : 0;
+    u4 x[kKB];
+    const int na = load_features(Y, mask, n_ch, pix, valid, lane, x);
+    if (!valid) continue;
+    const long row = dst + i;
+    uint2* xr = reinterpret_cast<uint2*>(D.x + row * kF);
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb) {
+      xr[(32 * kb + 4 * g) / 4] = make_uint2(x[kb].x, x[kb].y);
+      xr[(32 * kb + 16 + 4 * g) / 4] = make_uint2(x[kb].z, x[kb].w);
+    }
+    // labels, mask and sample weight: 16 lanes per pixel row
+    const float4* src = reinterpret_cast<const float4*>(weights + pix * n_ch);
+    float4* gd = reinterpret_cast<float4*>(D.g + row * kF);
+    for (int q = g; q < kF / 4; q += 4) gd[q] = src[q];
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) D.mask[row * 4 + q] = mask[pix * nw + q];
+      D.sw[row] = (status[pix] == 0 && na >= 4) ? 1.0f / static_cast<float>(na) : 0.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training step, forward and backward, one wavefront per 16 * kNT samples
+// with weight fragments read from global memory (L2-resident: 1 MB for both
+// layouts). Writes, for the weight-gradient kernel, the layer inputs H_l and
+// the output gradients dZ_l ([sample][feature], bf16), and the FP32
+// pre-activations for the GELU derivative; every store is a lane's row base
+// plus an immediate offset. Loss partial per wavefront.
+//
+// Loss: (1/B) sum_b sw_b sum_{c active} |1 + o_c - g_c|^2 (o, g complex).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t mix32(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return static_cast<uint32_t>(x);
+}
+
+__device__ __forceinline__ long sample_index(const ApodStep& S, int b) {
+  if (S.random) {
+    const uint64_t h = mix32(S.seed * 0x9e3779b97f4a7c15ull + static_cast<uint64_t>(S.step) * 0x100000001b3ull + b);
+    return static_cast<long>(h % static_cast<uint64_t>(S.n_data));
+  }
+  return b < S.n_data ? b : -1;
+}
+
+// A lane's B-operand fragments (8 bf16 per K block) into row `row` of a
+// [sample][256] bf16 buffer: features 32 kb + 4 g + 0..3 and 32 kb + 16 + 4 g + 0..3.
+__device__ __forceinline__ void store_frags(uint16_t* row, const u4 (&v)[kKB], int g) {
+  uint2* p = reinterpret_cast<uint2*>(row) + g;
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+    p[8 * kb] = make_uint2(v[kb].x, v[kb].y);
+    p[8 * kb + 4] = make_uint2(v[kb].z, v[kb].w);
+  }
+}
+
+__global__ __launch_bounds__(64) void apod_train_kernel(ApodNet N, ApodData D, ApodStep S,
+                                                        ApodWork W) {
+  const int lane = threadIdx.x, g = lane >> 4, r = lane & 15;
+  const int B = S.batch;
+  const int b0 = blockIdx.x * (16 * kNT);
+  const u4* wf = reinterpret_cast<const u4*>(N.wf);
+  const u4* wb = reinterpret_cast<const u4*>(N.wb);
+  long src[kNT];
+  u4 act[kKB][kNT];
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    src[t] = sample_index(S, b0 + 16 * t + r);
+    const uint2* xr = reinterpret_cast<const uint2*>(D.x + (src[t] < 0 ? 0 : src[t]) * kF) + g;
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb) {
+      uint2 lo = make_uint2(0u, 0u), hi = lo;
+      if (src[t] >= 0) {
+        lo = xr[8 * kb];
+        hi = xr[8 * kb + 4];
+      }
+      act[kb][t] = u4{lo.x, lo.y, hi.x, hi.y};
+    }
+  }
+  f4 acc[kMT][kNT];
+#pragma nounroll
+  for (int l = 0; l < kL; ++l) {
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      u4 v[kKB];
+#pragma unroll
+      for (int kb = 0; kb < kKB; ++kb) v[kb] = act[kb][t];
+      store_frags(W.h + (static_cast<size_t>(l) * B + b0 + 16 * t + r) * kF, v, g);
+    }
+    layer<true>(wf + l * (kLayerBf16 / 8), N.bias + l * kF, act, acc, lane);
+    if (l + 1 < kL) {
+#pragma unroll
+      for (int t = 0; t < kNT; ++t) {
+        float4* zr = reinterpret_cast<float4*>(W.z + (static_cast<size_t>(l) * B + b0 + 16 * t + r) * kF) + g;
+#pragma unroll
+        for (int m = 0; m < kMT; ++m) {
+          const f4 a = acc[m][t];
+          zr[4 * m] = make_float4(a[0], a[1], a[2], a[3]);
+        }
+      }
+      activate(acc, act);
+    }
+  }
+  // loss and dO (FP32); dZ is then rounded to bf16, the operand of the
+  // backward GEMMs and of the weight gradients.
+  float loss = 0.0f;
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const long s = src[t];
+    float sw = 0.0f;
+    uint32_t mw[4] = {0u, 0u, 0u, 0u};
+    if (s >= 0) {
+      sw = D.sw[s];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mw[q] = D.mask[s * 4 + q];
+    }
+    const float gs = S.grad_scale * sw;
+    const float4* lr = reinterpret_cast<const float4*>(D.g + (s < 0 ? 0 : s) * kF) + g;
+#pragma unroll
+    for (int m = 0; m < kMT; ++m) {
+      float4 lab = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (sw != 0.0f) lab = lr[4 * m];
+      const float lv[4] = {lab.x, lab.y, lab.z, lab.w};
+      const int ch = 8 * m + 2 * g;
+      const uint32_t bits = mw[ch >> 5] >> (ch & 31);
+      f4 d;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool on = (bits >> (i >> 1)) & 1u;
+        const float pred = acc[m][t][i] + ((i & 1) ? 0.0f : 1.0f);
+        const float e = on ? pred - lv[i] : 0.0f;
+        loss += sw * e * e;
+        d[i] = gs * e;
+      }
+      acc[m][t] = d;
+    }
+  }
+  // backward: dZ_l (acc) -> bf16 fragments -> dH_l = W_l dZ_l -> dZ_{l-1} = dH_l * gelu'(Z_{l-1})
+#pragma nounroll
+  for (int l = kL - 1; l >= 0; --l) {
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      u4 v[kKB];
+#pragma unroll
+      for (int kb = 0; kb < kKB; ++kb) {
+        const f4 a = acc[2 * kb][t], c = acc[2 * kb + 1][t];
+        v[kb] = u4{pack2(a[0], a[1]), pack2(a[2], a[3]), pack2(c[0], c[1]), pack2(c[2], c[3])};
+        act[kb][t] = v[kb];
+      }
+      store_frags(W.dz + (static_cast<size_t>(l) * B + b0 + 16 * t + r) * kF, v, g);
+    }
+    if (l == 0) break;
+    layer<true>(wb + l * (kLayerBf16 / 8), nullptr, act, acc, lane);
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      const float4* zr = reinterpret_cast<const float4*>(W.z + (static_cast<size_t>(l - 1) * B + b0 + 16 * t + r) * kF) + g;
+#pragma unroll
+      for (int m = 0; m < kMT; ++m) {
+        const float4 z = zr[4 * m];
+        acc[m][t][0] *= gelu_grad(z.x);
+        acc[m][t][1] *= gelu_grad(z.y);
+        acc[m][t][2] *= gelu_grad(z.z);
+        acc[m][t][3] *= gelu_grad(z.w);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o, 64);
+  if (lane == 0) W.loss_part[blockIdx.x] = loss * S.loss_scale;
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradients dW_l[in][out] = sum_b H_l[b][in] dZ_l[b][out] over one K
+// split of the batch, bf16 MFMA with the samples as K. Workgroup = 4
+// wavefronts on a 64 x 64 block (2 x 2 tiles each); per 32-sample step the
+// two 32 x 64 row blocks are staged transposed in LDS ([feature][sample]) so
+// each operand fragment is one 16-byte read. Grid (16 blocks, splits, 4).
+// ---------------------------------------------------------------------------
+constexpr int kDwPad = 40;  // bf16 per transposed LDS row (32 samples + 8)
+
+__global__ __launch_bounds__(256) void apod_dw_kernel(ApodWork W, int B, int ks) {
+  __shared__ __align__(16) uint16_t ht[64 * kDwPad], dt[64 * kDwPad];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int blk = blockIdx.x, split = blockIdx.y, l = blockIdx.z;
+  const int in0 = (blk >> 2) * 64, out0 = (blk & 3) * 64;
+  const int wi = (wv >> 1) * 32, wo = (wv & 1) * 32;
+  const uint16_t* h = W.h + static_cast<size_t>(l) * B * kF;
+  const uint16_t* dz = W.dz + static_cast<size_t>(l) * B * kF;
+  f4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  // staging: thread -> (sample sr, 8 features from fc)
+  const int sr = tid >> 3, fc = (tid & 7) * 8;
+  const int k_lo = split * ks, k_hi = k_lo + ks;
+  for (int k0 = k_lo; k0 < k_hi; k0 += 32) {
+    const u4 hv = *reinterpret_cast<const u4*>(h + static_cast<size_t>(k0 + sr) * kF + in0 + fc);
+    const u4 dv = *reinterpret_cast<const u4*>(dz + static_cast<size_t>(k0 + sr) * kF + out0 + fc);
+    __syncthreads();
+    const uint32_t he[4] = {hv.x, hv.y, hv.z, hv.w}, de[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      ht[(fc + e) * kDwPad + sr] = static_cast<uint16_t>(he[e >> 1] >> (16 * (e & 1)));
+      dt[(fc + e) * kDwPad + sr] = static_cast<uint16_t>(de[e >> 1] >> (16 * (e & 1)));
+    }
+    __syncthreads();
+    u4 av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) av[a] = *reinterpret_cast<const u4*>(ht + (wi + 16 * a + r) * kDwPad + 8 * g);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) bv[c] = *reinterpret_cast<const u4*>(dt + (wo + 16 * c + r) * kDwPad + 8 * g);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[a]), as_bf8(bv[c]), acc[a][c], 0, 0, 0);
+  }
+  float* p = W.partial + (static_cast<size_t>(l) * gridDim.y + split) * kLayerBf16;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        p[(in0 + wi + 16 * a + 4 * g + i) * kF + out0 + wo + 16 * c + r] = acc[a][c][i];
+}
+
+// Bias gradient partials db_l[chunk][out] = sum over the chunk's samples of
+// dZ_l[b][out]; grid (kDbChunks, 4), one thread per output feature.
+constexpr int kDbChunks = 32;
+
+__global__ __launch_bounds__(256) void apod_db_kernel(ApodWork W, int B) {
+  const int out = threadIdx.x, chunk = blockIdx.x, l = blockIdx.y;
+  const int per = (B + kDbChunks - 1) / kDbChunks;
+  const int lo = chunk * per, hi = min(B, lo + per);
+  const uint16_t* dz = W.dz + static_cast<size_t>(l) * B * kF + out;
+  float s = 0.0f;
+  for (int b = lo; b < hi; ++b) s += bf2f(dz[static_cast<size_t>(b) * kF]);
+  W.db[(l * kDbChunks + chunk) * kF + out] = s;
+}
+
+// Loss of the step: the wavefront partials summed in order.
+__global__ void apod_loss_kernel(const float* part, int n, float* out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += part[i];
+  *out = static_cast<float>(s);
+}
+
+// Gradient reduction over the K splits, then either the plain gradient
+// (grads != nullptr) or an Adam step on the FP32 master parameters; the
+// bf16 swizzled copies are rewritten from the master either way.
+__global__ __launch_bounds__(256) void apod_update_kernel(ApodNet N, ApodWork W, int splits,
+                                                          ApodAdam opt, float* grads) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kApodParams) return;
+  const int nw = kL * kLayerBf16;
+  float gr;
+  if (p < nw) {
+    const int l = p / kLayerBf16, e = p - l * kLayerBf16;
+    float s = 0.0f;
+    for (int k = 0; k < splits; ++k) s += W.partial[(static_cast<size_t>(l) * splits + k) * kLayerBf16 + e];
+    gr = s;
+  } else {
+    const int l = (p - nw) / kF, out = (p - nw) - l * kF;
+    float s = 0.0f;
+    for (int k = 0; k < kDbChunks; ++k) s += W.db[(l * kDbChunks + k) * kF + out];
+    gr = s;
+  }
+  if (grads) {
+    grads[p] = gr;
+    return;
+  }
+  float w = N.master[p];
+  const float m = opt.beta1 * N.m[p] + (1.0f - opt.beta1) * gr;
+  const float v = opt.beta2 * N.v[p] + (1.0f - opt.beta2) * gr * gr;
+  N.m[p] = m;
+  N.v[p] = v;
+  const float mh = m / opt.bc1, vh = v / opt.bc2;
+  const float decay = p < nw ? opt.weight_decay : 0.0f;
+  w -= opt.lr * (mh / (sqrtf(vh) + opt.eps) + decay * w);
+  N.master[p] = w;
+  if (p < nw) {
+    const int l = p / kLayerBf16, e = p - l * kLayerBf16;
+    const int in = e / kF, out = e - in * kF;
+    const uint16_t h = static_cast<uint16_t>(f2bf(w));
+    N.wf[l * kLayerBf16 + frag_pos(out, in)] = h;
+    N.wb[l * kLayerBf16 + frag_pos(in, out)] = h;
+  } else {
+    N.bias[p - nw] = w;
+  }
+}
+
+// The bf16 swizzled copies and the FP32 bias from the master parameters.
+__global__ __launch_bounds__(256) void apod_pack_kernel(ApodNet N) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kApodParams) return;
+  const int nw = kL * kLayerBf16;
+  const float w = N.master[p];
+  if (p < nw) {
+    const int l = p / kLayerBf16, e = p - l * kLayerBf16;
+    const int in = e / kF, out = e - in * kF;
+    const uint16_t h = static_cast<uint16_t>(f2bf(w));
+    N.wf[l * kLayerBf16 + frag_pos(out, in)] = h;
+    N.wb[l * kLayerBf16 + frag_pos(in, out)] = h;
+  } else {
+    N.bias[p - nw] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+size_t apod_infer_smem() { return sizeof(uint16_t) * kLayerBf16; }
+
+void launch_apod_pack(const ApodNet& N, hipStream_t st) {
+  hipLaunchKernelGGL(apod_pack_kernel, dim3((kApodParams + 255) / 256), dim3(256), 0, st, N);
+}
+
+void launch_apod_infer(const ApodNet& N, int n_pix, int n_ch, const float2* Y,
+                       const uint32_t* mask, float2* frame, float* acc, int cf, int incoherent,
+                       hipStream_t st) {
+  const int per_block = 8 * 16 * kNT;
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(apod_infer_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+      static_cast<int>(apod_infer_smem()));
+  (void)attr;
+  hipLaunchKernelGGL(apod_infer_kernel, dim3((n_pix + per_block - 1) / per_block), dim3(512),
+                     apod_infer_smem(), st, N, n_pix, n_ch, Y, mask, frame, acc, cf, incoherent);
+}
+
+void launch_apod_gather(int n_list, const int* pixels, int n_ch, const float2* Y,
+                        const uint32_t* mask, const float2* weights, const int* status,
+                        const ApodData& D, long dst, hipStream_t st) {
+  hipLaunchKernelGGL(apod_gather_kernel, dim3((n_list + 16 * kNT - 1) / (16 * kNT)), dim3(64), 0,
+                     st, n_list, pixels, n_ch, Y, mask, weights, status, D, dst);
+}
+
+int apod_splits(int batch) { return batch >= 256 ? batch / 256 : 1; }
+
+void launch_apod_step(const ApodNet& N, const ApodData& D, const ApodStep& S, const ApodWork& W,
+                      const ApodAdam& opt, float* grads, float* loss_out, hipStream_t st) {
+  const int B = S.batch;
+  const int nwaves = B / (16 * kNT);
+  hipLaunchKernelGGL(apod_train_kernel, dim3(nwaves), dim3(64), 0, st, N, D, S, W);
+  const int splits = apod_splits(B);
+  hipLaunchKernelGGL(apod_dw_kernel, dim3(16, splits, kL), dim3(256), 0, st, W, B, B / splits);
+  hipLaunchKernelGGL(apod_db_kernel, dim3(kDbChunks, kL), dim3(kF), 0, st, W, B);
+  hipLaunchKernelGGL(apod_loss_kernel, dim3(1), dim3(64), 0, st, W.loss_part, nwaves, loss_out);
+  hipLaunchKernelGGL(apod_update_kernel, dim3((kApodParams + 255) / 256), dim3(256), 0, st, N, W,
+                     splits, opt, grads);
+}
+
+}  // namespace usxd

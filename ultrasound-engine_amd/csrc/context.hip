@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
@@ -59,6 +60,14 @@ struct usx_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::map<std::string, std::pair<long, double>> totals;
+  // learned apodization: parameters, optimizer state, training set, scratch
+  bool apod_loaded = false;
+  int apod_t = 0;
+  DevBuf ap_master, ap_m, ap_v, ap_wf, ap_wb;
+  DevBuf ds_x, ds_g, ds_mask, ds_sw;
+  size_t ds_n = 0, ds_cap = 0;
+  DevBuf tr_h, tr_z, tr_dz, tr_partial, tr_db, tr_loss_part, tr_losses, tr_grads;
+  DevBuf mv_w, mv_status, ap_pixels;
 };
 
 namespace {
@@ -160,7 +169,10 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->mv_range_average < 0)
     fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_range_average must be >= 0");
   if (b->rx_window < 0 || b->rx_window > 4) fail(c, USX_ERR_INVALID, "unknown window");
-  if (b->combiner < 0 || b->combiner > 2) fail(c, USX_ERR_INVALID, "unknown combiner");
+  if (b->combiner < 0 || b->combiner > 3) fail(c, USX_ERR_INVALID, "unknown combiner");
+  if (b->combiner == 3 && b->n_channels != USX_APOD_CHANNELS)
+    fail(c, USX_ERR_INVALID, "learned apodization: the network takes %d channels, got %d",
+         USX_APOD_CHANNELS, b->n_channels);
   if (b->compounding < 0 || b->compounding > 2) fail(c, USX_ERR_INVALID, "unknown compounding");
   if (b->is_iq && !(b->demodulation_frequency > 0))
     fail(c, USX_ERR_INVALID, "IQ ChannelData needs a positive demodulation_frequency");
@@ -259,8 +271,22 @@ BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
 
 constexpr int kSlots = 16;
 
+usxd::ApodNet apod_net(usx_ctx* c) {
+  usxd::ApodNet N{};
+  N.master = static_cast<float*>(c->ap_master.p);
+  N.m = static_cast<float*>(c->ap_m.p);
+  N.v = static_cast<float*>(c->ap_v.p);
+  N.wf = static_cast<uint16_t*>(c->ap_wf.p);
+  N.wb = static_cast<uint16_t*>(c->ap_wb.p);
+  N.bias = N.master ? N.master + usxd::kApodLayers * usxd::kApodFeatures * usxd::kApodFeatures : nullptr;
+  return N;
+}
+
 // Enqueue the beamforming of device-resident data into a device frame.
-void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* d_frame) {
+// d_weights / d_status (MV only, may be null): the effective channel weights
+// and per-pixel status of the minimum-variance solve.
+void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* d_frame,
+                  float2* d_weights = nullptr, int* d_status = nullptr) {
   const BfArgs A = make_args(c, b);
   const bool iq = b->is_iq != 0;
   const int n_pix = b->n_axis0 * b->n_axis1;
@@ -290,10 +316,23 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
   M.cf = cf;
   M.precision = b->mv_precision;
   M.refine = b->mv_refine;
+  if (comb == 3 && !c->apod_loaded)
+    fail(c, USX_ERR_INVALID, "learned apodization: no network loaded (usx_apod_set_params)");
   auto mv = [&](float2* Y, uint32_t* mk, float* acc) {
+    if (comb == 3) {
+      timed(c, "apod_infer", [&] {
+        usxd::launch_apod_infer(apod_net(c), n_pix, b->n_channels, Y, mk, d_frame, acc, cf,
+                                acc != nullptr, st);
+      });
+      return;
+    }
     const int nb = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray) / 16;
     int* counts = static_cast<int*>(ensure(c, c->mv_counts, 16 * sizeof(int)));
     int* lists = static_cast<int*>(ensure(c, c->mv_lists, static_cast<size_t>(nb) * n_pix * sizeof(int)));
+    M.weights = d_weights;
+    M.status = d_status;
+    if (d_weights)
+      timed(c, "fill_zero", [&] { usxd::launch_fill_zero(reinterpret_cast<float*>(d_weights), 2 * ny, st); });
     timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, counts, lists, st); });
   };
 
@@ -303,8 +342,8 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
     float2* Y = Ybuf();
     uint32_t* mk = Mbuf();
     for (int e = b->event_begin; e < b->event_end; ++e) {
-      timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e, 1, 0, 1, comb == 2, iq, st); });
-      if (comb == 2) {
+      timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e, 1, 0, 1, comb >= 2, iq, st); });
+      if (comb >= 2) {
         M.incoherent = 1;
         mv(Y, mk, acc);
       } else {
@@ -315,7 +354,7 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
     return;
   }
 
-  if (comb == 2) {
+  if (comb >= 2) {
     float2* Y = Ybuf();
     uint32_t* mk = Mbuf();
     if (select) {
@@ -369,6 +408,51 @@ void check_launch(usx_ctx* c, const char* what) { hip_check(c, hipGetLastError()
 
 
 
+// ---- learned apodization helpers -------------------------------------------
+
+void apod_alloc(usx_ctx* c) {
+  const size_t np = usxd::kApodParams;
+  const size_t nw = static_cast<size_t>(usxd::kApodLayers) * usxd::kApodFeatures * usxd::kApodFeatures;
+  ensure(c, c->ap_master, np * sizeof(float));
+  ensure(c, c->ap_m, np * sizeof(float));
+  ensure(c, c->ap_v, np * sizeof(float));
+  ensure(c, c->ap_wf, nw * sizeof(uint16_t));
+  ensure(c, c->ap_wb, nw * sizeof(uint16_t));
+}
+
+// Training scratch for batches of up to `batch` rows.
+usxd::ApodWork apod_work(usx_ctx* c, int batch) {
+  const size_t B = static_cast<size_t>(batch), F = usxd::kApodFeatures;
+  const int splits = usxd::apod_splits(batch);
+  usxd::ApodWork W{};
+  W.h = static_cast<uint16_t*>(ensure(c, c->tr_h, 4 * B * F * sizeof(uint16_t)));
+  W.z = static_cast<float*>(ensure(c, c->tr_z, 3 * B * F * sizeof(float)));
+  W.dz = static_cast<uint16_t*>(ensure(c, c->tr_dz, 4 * B * F * sizeof(uint16_t)));
+  W.partial = static_cast<float*>(ensure(c, c->tr_partial, 4 * static_cast<size_t>(splits) * F * F * sizeof(float)));
+  W.db = static_cast<float*>(ensure(c, c->tr_db, 4 * 32 * F * sizeof(float)));
+  W.loss_part = static_cast<float*>(ensure(c, c->tr_loss_part, (B / 16 + 1) * sizeof(float)));
+  return W;
+}
+
+usxd::ApodData apod_data(usx_ctx* c) {
+  usxd::ApodData D{};
+  D.x = static_cast<uint16_t*>(c->ds_x.p);
+  D.g = static_cast<float*>(c->ds_g.p);
+  D.mask = static_cast<uint32_t*>(c->ds_mask.p);
+  D.sw = static_cast<float*>(c->ds_sw.p);
+  return D;
+}
+
+void apod_reserve(usx_ctx* c, size_t cap) {
+  const size_t F = usxd::kApodFeatures;
+  ensure(c, c->ds_x, cap * F * sizeof(uint16_t));
+  ensure(c, c->ds_g, cap * F * sizeof(float));
+  ensure(c, c->ds_mask, cap * 4 * sizeof(uint32_t));
+  ensure(c, c->ds_sw, cap * sizeof(float));
+  c->ds_cap = cap;
+  c->ds_n = 0;
+}
+
 size_t data_bytes(const usx_bf_desc* b) {
   return static_cast<size_t>(b->n_events) * b->n_channels * b->n_samples *
          (b->is_iq ? sizeof(usx_c32) : sizeof(float));
@@ -414,7 +498,11 @@ usx_status usx_ctx_destroy(usx_ctx* c) {
   }
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->in, &c->out, &c->iq, &c->Y, &c->mask, &c->acc, &c->mv_counts, &c->mv_lists})
+  for (DevBuf* b : {&c->in, &c->out, &c->iq, &c->Y, &c->mask, &c->acc, &c->mv_counts, &c->mv_lists,
+                    &c->ap_master, &c->ap_m, &c->ap_v, &c->ap_wf, &c->ap_wb, &c->ds_x, &c->ds_g,
+                    &c->ds_mask, &c->ds_sw, &c->tr_h, &c->tr_z, &c->tr_dz, &c->tr_partial,
+                    &c->tr_db, &c->tr_loss_part, &c->tr_losses, &c->tr_grads, &c->mv_w,
+                    &c->mv_status, &c->ap_pixels})
     if (b->p) (void)hipFree(b->p);
   for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0})
     if (t->dev.p) (void)hipFree(t->dev.p);
@@ -586,4 +674,213 @@ usx_status usx_process_frame(usx_ctx* c, const usx_demod_desc* d, const usx_bf_d
   });
 }
 
-}  // extern "C"
+usx_status usx_mv_weights(usx_ctx* c, const usx_bf_desc* b, const void* data, usx_c32* frame,
+                          usx_c32* weights, int* status, int memory) {
+  return guarded(c, [&] {
+    check_bf(c, b);
+    if (b->combiner != 2) fail(c, USX_ERR_INVALID, "usx_mv_weights: combiner must be MV");
+    if (b->compounding == 1) fail(c, USX_ERR_INVALID, "usx_mv_weights: incoherent compounding has no weights");
+    if (!data || !frame || !weights) fail(c, USX_ERR_INVALID, "usx_mv_weights: null data pointer");
+    const size_t npix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+    const size_t nw = npix * b->n_channels;
+    if (memory == USX_DEVICE) {
+      run_beamform(c, b, data, reinterpret_cast<float2*>(frame), reinterpret_cast<float2*>(weights), status);
+      check_launch(c, "mv_weights launch");
+      return;
+    }
+    void* d_in = ensure(c, c->in, data_bytes(b));
+    float2* d_out = static_cast<float2*>(ensure(c, c->out, npix * sizeof(float2)));
+    float2* d_w = static_cast<float2*>(ensure(c, c->mv_w, nw * sizeof(float2)));
+    int* d_st = static_cast<int*>(ensure(c, c->mv_status, npix * sizeof(int)));
+    hip_check(c, hipMemcpyAsync(d_in, data, data_bytes(b), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(data)");
+    run_beamform(c, b, d_in, d_out, d_w, d_st);
+    check_launch(c, "mv_weights launch");
+    hip_check(c, hipMemcpyAsync(frame, d_out, npix * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(frame)");
+    hip_check(c, hipMemcpyAsync(weights, d_w, nw * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(weights)");
+    if (status)
+      hip_check(c, hipMemcpyAsync(status, d_st, npix * sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(status)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_apod_set_params(usx_ctx* c, const float* params) {
+  return guarded(c, [&] {
+    if (!params) fail(c, USX_ERR_INVALID, "usx_apod_set_params: null parameters");
+    apod_alloc(c);
+    const size_t bytes = sizeof(float) * usxd::kApodParams;
+    hip_check(c, hipMemcpyAsync(c->ap_master.p, params, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(params)");
+    usxd::launch_fill_zero(static_cast<float*>(c->ap_m.p), usxd::kApodParams, c->stream);
+    usxd::launch_fill_zero(static_cast<float*>(c->ap_v.p), usxd::kApodParams, c->stream);
+    usxd::launch_apod_pack(apod_net(c), c->stream);
+    check_launch(c, "apod pack launch");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->apod_loaded = true;
+    c->apod_t = 0;
+  });
+}
+
+usx_status usx_apod_get_params(usx_ctx* c, float* params) {
+  return guarded(c, [&] {
+    if (!params) fail(c, USX_ERR_INVALID, "usx_apod_get_params: null buffer");
+    if (!c->apod_loaded) fail(c, USX_ERR_INVALID, "learned apodization: no network loaded");
+    hip_check(c, hipMemcpyAsync(params, c->ap_master.p, sizeof(float) * usxd::kApodParams,
+                                hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(params)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_apod_dataset_reserve(usx_ctx* c, size_t capacity) {
+  return guarded(c, [&] {
+    if (capacity == 0) fail(c, USX_ERR_INVALID, "usx_apod_dataset_reserve: capacity must be > 0");
+    apod_reserve(c, capacity);
+  });
+}
+
+usx_status usx_apod_dataset_size(usx_ctx* c, size_t* n) {
+  return guarded(c, [&] {
+    if (!n) fail(c, USX_ERR_INVALID, "usx_apod_dataset_size: null output");
+    *n = c->ds_n;
+  });
+}
+
+usx_status usx_apod_dataset_append(usx_ctx* c, const usx_demod_desc* d, const usx_bf_desc* b,
+                                   const void* data, int memory, const int* pixels, int n_pixels,
+                                   usx_c32* frame) {
+  return guarded(c, [&] {
+    check_bf(c, b);
+    if (d) check_demod(c, d);
+    if (b->combiner != 2) fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: labels come from MV (combiner 2)");
+    if (b->compounding == 1) fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: incoherent compounding has no weights");
+    if (b->n_channels != USX_APOD_CHANNELS)
+      fail(c, USX_ERR_INVALID, "learned apodization: the network takes %d channels, got %d",
+           USX_APOD_CHANNELS, b->n_channels);
+    if (!data || !pixels || n_pixels < 0) fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: null data or pixel list");
+    if (c->ds_n + static_cast<size_t>(n_pixels) > c->ds_cap)
+      fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: %zu + %d rows exceed the reserved %zu",
+           c->ds_n, n_pixels, c->ds_cap);
+    const int npix = b->n_axis0 * b->n_axis1;
+    for (int i = 0; i < n_pixels; ++i)
+      if (pixels[i] < 0 || pixels[i] >= npix)
+        fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: pixel %d outside the %d-pixel grid", pixels[i], npix);
+    if (d && (!b->is_iq || b->n_events * b->n_channels != d->n_traces || b->n_samples != d->n_out))
+      fail(c, USX_ERR_INVALID, "usx_apod_dataset_append: demodulate and beamform shapes disagree");
+    const void* d_data = data;
+    if (memory != USX_DEVICE) {
+      const size_t bytes = d ? sizeof(float) * static_cast<size_t>(d->n_traces) * d->n_in : data_bytes(b);
+      void* p = ensure(c, c->in, bytes);
+      hip_check(c, hipMemcpyAsync(p, data, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(data)");
+      d_data = p;
+    }
+    if (d) {
+      const size_t nout = static_cast<size_t>(d->n_traces) * d->n_out;
+      float2* d_iq = static_cast<float2*>(ensure(c, c->iq, nout * sizeof(float2)));
+      run_demod(c, d, static_cast<const float*>(d_data), d_iq);
+      d_data = d_iq;
+    }
+    float2* d_out = static_cast<float2*>(ensure(c, c->out, npix * sizeof(float2)));
+    float2* d_w = static_cast<float2*>(ensure(c, c->mv_w, static_cast<size_t>(npix) * b->n_channels * sizeof(float2)));
+    int* d_st = static_cast<int*>(ensure(c, c->mv_status, npix * sizeof(int)));
+    run_beamform(c, b, d_data, d_out, d_w, d_st);
+    int* d_pix = static_cast<int*>(ensure(c, c->ap_pixels, sizeof(int) * (n_pixels + 1)));
+    hip_check(c, hipMemcpyAsync(d_pix, pixels, sizeof(int) * n_pixels, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(pixels)");
+    if (n_pixels > 0)
+      timed(c, "apod_gather", [&] {
+        usxd::launch_apod_gather(n_pixels, d_pix, b->n_channels, static_cast<const float2*>(c->Y.p),
+                                 static_cast<const uint32_t*>(c->mask.p), d_w, d_st, apod_data(c),
+                                 static_cast<long>(c->ds_n), c->stream);
+      });
+    check_launch(c, "apod dataset launch");
+    if (frame)
+      hip_check(c, hipMemcpyAsync(frame, d_out, npix * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(frame)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->ds_n += n_pixels;
+  });
+}
+
+usx_status usx_apod_train(usx_ctx* c, const usx_apod_train_desc* t, float* losses) {
+  return guarded(c, [&] {
+    if (!t) fail(c, USX_ERR_INVALID, "usx_apod_train: null descriptor");
+    if (!c->apod_loaded) fail(c, USX_ERR_INVALID, "learned apodization: no network loaded");
+    if (c->ds_n == 0) fail(c, USX_ERR_INVALID, "usx_apod_train: the training set is empty");
+    if (t->batch < 256 || t->batch % 256 != 0 || t->batch > (1 << 20))
+      fail(c, USX_ERR_INVALID, "usx_apod_train: batch must be a multiple of 256 up to 2^20");
+    if (t->n_steps < 0) fail(c, USX_ERR_INVALID, "usx_apod_train: n_steps must be >= 0");
+    const usxd::ApodWork W = apod_work(c, t->batch);
+    float* d_losses = static_cast<float*>(ensure(c, c->tr_losses, sizeof(float) * (t->n_steps + 1)));
+    const usxd::ApodNet N = apod_net(c);
+    const usxd::ApodData D = apod_data(c);
+    for (int k = 0; k < t->n_steps; ++k) {
+      const int step = ++c->apod_t;
+      usxd::ApodStep S{};
+      S.batch = t->batch;
+      S.n_data = static_cast<long>(c->ds_n);
+      S.random = 1;
+      S.seed = t->seed;
+      S.step = step;
+      S.grad_scale = 2.0f / static_cast<float>(t->batch);
+      S.loss_scale = 1.0f / static_cast<float>(t->batch);
+      usxd::ApodAdam opt{};
+      opt.lr = t->lr;
+      opt.beta1 = t->beta1;
+      opt.beta2 = t->beta2;
+      opt.eps = t->eps;
+      opt.weight_decay = t->weight_decay;
+      opt.bc1 = static_cast<float>(1.0 - std::pow(static_cast<double>(t->beta1), step));
+      opt.bc2 = static_cast<float>(1.0 - std::pow(static_cast<double>(t->beta2), step));
+      timed(c, "apod_step", [&] { usxd::launch_apod_step(N, D, S, W, opt, nullptr, d_losses + k, c->stream); });
+    }
+    check_launch(c, "apod train launch");
+    if (losses && t->n_steps > 0)
+      hip_check(c, hipMemcpyAsync(losses, d_losses, sizeof(float) * t->n_steps, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(losses)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_apod_gradient(usx_ctx* c, int n, const uint16_t* x, const float* g,
+                             const uint32_t* mask, const float* sw, float* loss, float* grads) {
+  return guarded(c, [&] {
+    if (!c->apod_loaded) fail(c, USX_ERR_INVALID, "learned apodization: no network loaded");
+    if (n <= 0 || n > 4096) fail(c, USX_ERR_INVALID, "usx_apod_gradient: n must be in 1..4096");
+    if (!x || !g || !mask || !sw || !loss || !grads) fail(c, USX_ERR_INVALID, "usx_apod_gradient: null pointer");
+    // the batch becomes a temporary training set (the stored one is kept)
+    const size_t F = usxd::kApodFeatures;
+    const int B = n <= 256 ? (n + 31) / 32 * 32 : (n + 255) / 256 * 256;
+    DevBuf tx{}, tg{}, tm{}, ts{};
+    auto cleanup = [&] {
+      for (DevBuf* p : {&tx, &tg, &tm, &ts})
+        if (p->p) (void)hipFree(p->p);
+    };
+    try {
+      usxd::ApodData D{};
+      D.x = static_cast<uint16_t*>(ensure(c, tx, n * F * sizeof(uint16_t)));
+      D.g = static_cast<float*>(ensure(c, tg, n * F * sizeof(float)));
+      D.mask = static_cast<uint32_t*>(ensure(c, tm, n * 4 * sizeof(uint32_t)));
+      D.sw = static_cast<float*>(ensure(c, ts, n * sizeof(float)));
+      hip_check(c, hipMemcpyAsync(D.x, x, n * F * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(x)");
+      hip_check(c, hipMemcpyAsync(D.g, g, n * F * sizeof(float), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(g)");
+      hip_check(c, hipMemcpyAsync(D.mask, mask, n * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(mask)");
+      hip_check(c, hipMemcpyAsync(D.sw, sw, n * sizeof(float), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(sw)");
+      const usxd::ApodWork W = apod_work(c, B);
+      float* d_grads = static_cast<float*>(ensure(c, c->tr_grads, sizeof(float) * (usxd::kApodParams + 1)));
+      float* d_loss = d_grads + usxd::kApodParams;
+      usxd::ApodStep S{};
+      S.batch = B;
+      S.n_data = n;
+      S.random = 0;
+      S.grad_scale = 2.0f / static_cast<float>(n);
+      S.loss_scale = 1.0f / static_cast<float>(n);
+      usxd::ApodAdam opt{};
+      usxd::launch_apod_step(apod_net(c), D, S, W, opt, d_grads, d_loss, c->stream);
+      check_launch(c, "apod gradient launch");
+      hip_check(c, hipMemcpyAsync(grads, d_grads, sizeof(float) * usxd::kApodParams, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(grads)");
+      hip_check(c, hipMemcpyAsync(loss, d_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(loss)");
+      hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
+}  // extern "C\"

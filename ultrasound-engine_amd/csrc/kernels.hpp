@@ -53,6 +53,14 @@ struct MvArgs {
   int incoherent;
   int precision;   // 0 FP32, 1 BF16, 2 FP16 covariance inputs
   int refine;      // one step of iterative refinement (FP64 residual)
+  // Optional outputs (coherent compounding only; nullptr when unused):
+  // weights[pix][ch] the effective per-channel weights g with
+  // out = sum_ch g_ch Y_ch before the coherence factor (the caller zeroes
+  // the buffer: only active channels are written); status[pix] 0 solved,
+  // 1 fewer than 4 active channels (DAS), 2 solve failed (DAS), 3 no
+  // active channel.
+  float2* weights;
+  int* status;
 };
 // counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
@@ -61,5 +69,60 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
 int mv_max_bucket_p(int n_ch, int subarray);
 size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P);
 constexpr int kMvMaxP = 64;
+
+// apod.hip: learned apodization MLP (256 -> 3 x 256 GELU -> 256, bf16 MFMA).
+constexpr int kApodFeatures = 256;
+constexpr int kApodLayers = 4;
+constexpr int kApodParams = kApodLayers * kApodFeatures * kApodFeatures + kApodLayers * kApodFeatures;
+constexpr int kApodChannels = kApodFeatures / 2;
+
+struct ApodNet {
+  float* master;   // kApodParams: W_0..W_3 ([in][out]), then b_0..b_3
+  float* m;        // Adam moments
+  float* v;
+  uint16_t* wf;    // bf16 forward fragments, 4 x 65536
+  uint16_t* wb;    // bf16 backward fragments, 4 x 65536
+  float* bias;     // = master + 4 * 65536
+};
+struct ApodData {  // training rows
+  uint16_t* x;     // [n][256] bf16 network input
+  float* g;        // [n][256] minimum-variance weights (re, im)
+  uint32_t* mask;  // [n][4] active channels
+  float* sw;       // [n] 1 / n_active, or 0 (excluded)
+};
+struct ApodStep {
+  int batch;       // multiple of 32 (and of 256 above 256)
+  long n_data;
+  int random;      // 1: rows drawn by hash(seed, step, b); 0: rows 0..n_data-1, then padding
+  uint64_t seed;
+  int step;
+  float grad_scale;  // 2 / rows (rows = batch, or the valid rows of a padded batch)
+  float loss_scale;  // 1 / rows
+};
+struct ApodWork {
+  uint16_t* h;     // [4][batch][256] layer inputs (bf16)
+  float* z;        // [3][batch][256] pre-activations
+  uint16_t* dz;    // [4][batch][256] layer output gradients (bf16)
+  float* partial;  // [4][splits][65536] weight-gradient partials
+  float* db;       // [4][32][256] bias-gradient partials
+  float* loss_part;  // [batch / 16] loss partials
+};
+struct ApodAdam {
+  float lr, beta1, beta2, eps, weight_decay;
+  float bc1, bc2;  // 1 - beta^t
+};
+size_t apod_infer_smem();
+int apod_splits(int batch);
+void launch_apod_pack(const ApodNet& N, hipStream_t st);
+void launch_apod_infer(const ApodNet& N, int n_pix, int n_ch, const float2* Y,
+                       const uint32_t* mask, float2* frame, float* acc, int cf, int incoherent,
+                       hipStream_t st);
+void launch_apod_gather(int n_list, const int* pixels, int n_ch, const float2* Y,
+                        const uint32_t* mask, const float2* weights, const int* status,
+                        const ApodData& D, long dst, hipStream_t st);
+// One training step (or, with grads != nullptr, only the gradient of the
+// batch into grads[kApodParams]); the step's loss goes to *loss_out (device).
+void launch_apod_step(const ApodNet& N, const ApodData& D, const ApodStep& S, const ApodWork& W,
+                      const ApodAdam& opt, float* grads, float* loss_out, hipStream_t st);
 
 }  // namespace usxd

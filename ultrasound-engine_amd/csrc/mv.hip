@@ -141,6 +141,13 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   if (valid)
     for (int w = 0; w < nw; ++w) na += __popc(mp[w]);
   if (valid && na == 0 && !M.incoherent) frame[pix] = make_float2(0.0f, 0.0f);
+  if (valid && M.status) M.status[pix] = na == 0 ? 3 : (na < 4 ? 1 : 0);
+  if (valid && na > 0 && na < 4 && M.weights) {
+    float2* gw = M.weights + static_cast<size_t>(pix) * M.n_ch;
+    for (int w = 0; w < nw; ++w)
+      for (uint32_t bits = mp[w]; bits; bits &= bits - 1)
+        gw[w * 32 + __ffs(bits) - 1] = make_float2(1.0f, 0.0f);
+  }
   if (valid && na > 0 && na < 4) {
     const float2* yc = Y + static_cast<size_t>(pix) * M.n_ch;
     float sr = 0.0f, si = 0.0f, inc = 0.0f;
@@ -377,6 +384,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       __syncthreads();
     }
     float2 out;
+    bool das = !ok;
     if (!ok) {
       float sr = 0.0f, si = 0.0f;
       for (int a = lane; a < na; a += 64) {
@@ -461,6 +469,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       orr = wsum(orr);
       oi = wsum(oi);
       if (hypotf(dr, di) < 1e-30f) {
+        das = true;
         float sr = 0.0f, si = 0.0f;
         for (int a = lane; a < na; a += 64) {
           sr += yc[act[a]].x;
@@ -473,7 +482,41 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
         out = make_float2(
             static_cast<float>((static_cast<double>(orr) * dr + static_cast<double>(oi) * di) / dd) * nf,
             static_cast<float>((static_cast<double>(oi) * dr - static_cast<double>(orr) * di) / dd) * nf);
+        if (M.weights) {
+          // g_c = n / (n_sub sum w) * sum_{a = max(0, c - n_sub + 1)}^{min(L - 1, c)} conj(w_a):
+          // inclusive prefix sums of conj(w) (lane a < L) through a wave scan.
+          float pr = lane < L ? w.x : 0.0f, pi = lane < L ? -w.y : 0.0f;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const float qr = __shfl_up(pr, o, 64), qi = __shfl_up(pi, o, 64);
+            if (lane >= o) {
+              pr += qr;
+              pi += qi;
+            }
+          }
+          if (lane < P) rv[lane] = make_float2(pr, pi);
+          __syncthreads();
+          const double sc = static_cast<double>(na) / (static_cast<double>(ns) * dd);
+          float2* gw = M.weights + static_cast<size_t>(pix) * n_ch;
+          for (int c = lane; c < na; c += 64) {
+            float2 acc = rv[min(L - 1, c)];
+            if (c - ns >= 0) {
+              const float2 lo = rv[c - ns];
+              acc.x -= lo.x;
+              acc.y -= lo.y;
+            }
+            // acc / (dr + j di) * n / n_sub
+            gw[act[c]] = make_float2(
+                static_cast<float>((static_cast<double>(acc.x) * dr + static_cast<double>(acc.y) * di) * sc),
+                static_cast<float>((static_cast<double>(acc.y) * dr - static_cast<double>(acc.x) * di) * sc));
+          }
+        }
       }
+    }
+    if (M.status && lane == 0 && das) M.status[pix] = 2;
+    if (M.weights && das) {
+      float2* gw = M.weights + static_cast<size_t>(pix) * n_ch;
+      for (int a = lane; a < na; a += 64) gw[act[a]] = make_float2(1.0f, 0.0f);
     }
     if (M.cf) {
       const float f = coherence(yc, act, na, lane);
