@@ -249,6 +249,39 @@ usx_status usx_apod_train(usx_ctx* ctx, const usx_apod_train_desc* t, float* los
 usx_status usx_apod_gradient(usx_ctx* ctx, int n, const uint16_t* x, const float* g,
                              const uint32_t* mask, const float* sw, float* loss, float* grads);
 
+/* ---- point-scatterer simulator (simulator.cpp:213-419) ------------------ */
+
+/* Everything Simulator::simulate reads besides the phantom, precomputed on
+ * the host with the reference's expressions (pulse(), element positions,
+ * transmit delays, attenuation in dB/m at the centre frequency, the trim
+ * and noise amplitudes). */
+typedef struct {
+  int n_elements, n_samples, n_events;
+  float sampling_frequency, t0, prf, speed_of_sound, wavelength, element_width;
+  int oversampling;
+  const float* element_x;         /* host, n_elements */
+  const float* delays;            /* host, n_events * n_elements (Transmit::delays) */
+  const float* apodization;       /* host, n_events * n_elements; 0 = element does not fire */
+  const int* has_apodization;     /* host, n_events; 0: every element fires with weight 1 */
+  int pulse_n;
+  const float* pulse;             /* host, pulse_n (Simulator::pulse) */
+  float pulse_half;               /* its half duration, s */
+  float attenuation;              /* dB per metre at the centre frequency; 0 = off */
+  float max_window;               /* max_incident_window_us * 1e-6 */
+  float trim_amp;                 /* 10^(incident_trim_db / 20) */
+  int spreading, directivity, motion, full_diffraction;
+  float noise_amp;                /* 10^(noise_db / 20), 0 = no noise */
+  const float* noise;             /* host, n_events * n_elements * n_samples standard
+                                     normals: std::normal_distribution<float>(0, 1) drawn
+                                     from std::mt19937(noise_seed + event); NULL if no noise */
+} usx_sim_desc;
+
+/* scatterers: host, n_scatterers * 7 floats (x, y, z, amplitude, vx, vy, vz);
+ * rf: n_events * n_elements * n_samples floats. Matches the host simulator
+ * to float rounding (device powf/sinf in the amplitudes), deterministically. */
+usx_status usx_simulate(usx_ctx* ctx, const usx_sim_desc* s, int n_scatterers,
+                        const float* scatterers, float* rf, int memory);
+
 #ifdef __cplusplus
 }
 #endif

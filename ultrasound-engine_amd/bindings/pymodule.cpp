@@ -129,6 +129,7 @@ class DeviceRunner {
   }
   bool is_rf() const { return is_rf_; }
   int iq_samples() const { return bplan_.desc.n_samples; }
+  int frame_pixels() const { return bplan_.desc.n_axis0 * bplan_.desc.n_axis1; }
 
   // ---- learned apodization (no reference counterpart) ----------------------
   using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
@@ -600,6 +601,8 @@ PYBIND11_MODULE(_usx, m) {
       .def_static("repeat", &Simulator::repeat, py::arg("transmits"), py::arg("n_ensemble"))
       .def("simulate", &Simulator::simulate, py::arg("transmits"), py::arg("phantom"),
            py::call_guard<py::gil_scoped_release>())
+      .def("simulate_gpu", &Simulator::simulate_gpu, py::arg("transmits"), py::arg("phantom"),
+           py::call_guard<py::gil_scoped_release>())
       .def_property_readonly("probe", py::overload_cast<>(&Simulator::probe, py::const_))
       .def_property_readonly("medium", py::overload_cast<>(&Simulator::medium, py::const_))
       .def_property_readonly("acquisition",
@@ -761,6 +764,7 @@ PYBIND11_MODULE(_usx, m) {
       .def_property_readonly("input_shape", &DeviceRunner::input_shape)
       .def_property_readonly("is_rf", &DeviceRunner::is_rf)
       .def_property_readonly("iq_samples", &DeviceRunner::iq_samples)
+      .def_property_readonly("frame_pixels", &DeviceRunner::frame_pixels)
       .def("set_apodization", &DeviceRunner::set_apodization, py::arg("params"))
       .def("get_apodization", &DeviceRunner::get_apodization)
       .def("mv_weights", &DeviceRunner::mv_weights, py::arg("channel_data"))

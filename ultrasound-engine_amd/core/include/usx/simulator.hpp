@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "usx/types.hpp"
@@ -68,6 +69,11 @@ class Simulator {
   static std::vector<Transmit> repeat(const std::vector<Transmit>& txs, int n_ensemble);
 
   ChannelData simulate(const std::vector<Transmit>& transmits, const Phantom& phantom) const;
+  // The same model on the GPU (no reference counterpart): equal to
+  // simulate() up to float rounding in the path amplitudes (device powf and
+  // sinf), deterministic, and orders of magnitude faster. Runs on the
+  // calling thread's GPU context.
+  ChannelData simulate_gpu(const std::vector<Transmit>& transmits, const Phantom& phantom) const;
 
   const Probe& probe() const { return probe_; }
   const Medium& medium() const { return medium_; }
@@ -82,6 +88,15 @@ class Simulator {
   Medium medium_;
   Acquisition acq_;
   SimulatorConfig config_;
+  // Receive-noise standard normals per event (simulate_gpu), kept while the
+  // seed and sizes stay the same.
+  struct NoiseTable {
+    std::uint32_t seed = 0;
+    int n_events = -1;
+    std::size_t n_per_event = 0;
+    std::vector<real> z;
+  };
+  mutable std::shared_ptr<NoiseTable> noise_;
 };
 
 }  // namespace usx

@@ -19,8 +19,10 @@
 #include <algorithm>
 #include <cmath>
 #include <random>
+#include <stdexcept>
 
 #include "usx/geometry.hpp"
+#include "gpu.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -369,6 +371,101 @@ ChannelData Simulator::simulate(const std::vector<Transmit>& transmits,
       }
     }
   }
+  return cd;
+}
+
+ChannelData Simulator::simulate_gpu(const std::vector<Transmit>& transmits,
+                                    const Phantom& phantom) const {
+  ChannelData cd;
+  cd.probe = probe_;
+  cd.medium = medium_;
+  cd.acq = acq_;
+  cd.transmits = transmits;
+  cd.is_iq = false;
+  cd.allocate();
+  const int n_ch = probe_.n_elements;
+  const int n_smp = acq_.n_samples;
+  const int n_ev = static_cast<int>(transmits.size());
+  real pulse_half = 0.0f;
+  const std::vector<real> pw = pulse(&pulse_half);
+  std::vector<real> ex(n_ch), delays(static_cast<std::size_t>(n_ev) * n_ch, 0.0f),
+      apod(static_cast<std::size_t>(n_ev) * n_ch, 0.0f);
+  std::vector<int> has_apod(std::max(n_ev, 1), 0);
+  for (int i = 0; i < n_ch; ++i) ex[i] = probe_.element_position(i).x;
+  for (int e = 0; e < n_ev; ++e) {
+    const Transmit& tx = transmits[e];
+    if (static_cast<int>(tx.delays.size()) != n_ch)
+      throw std::invalid_argument("simulate: transmit delays must have one entry per element");
+    std::copy(tx.delays.begin(), tx.delays.end(), delays.begin() + static_cast<std::ptrdiff_t>(e) * n_ch);
+    if (!tx.apodization.empty()) {
+      if (static_cast<int>(tx.apodization.size()) != n_ch)
+        throw std::invalid_argument("simulate: transmit apodization must have one entry per element");
+      has_apod[e] = 1;
+      std::copy(tx.apodization.begin(), tx.apodization.end(),
+                apod.begin() + static_cast<std::ptrdiff_t>(e) * n_ch);
+    }
+  }
+  const real f0_mhz = probe_.center_frequency * 1e-6f;
+  const bool noise = config_.noise_db > -200.0f;
+  const std::size_t n_tot = static_cast<std::size_t>(n_ch) * n_smp;
+  if (noise && (!noise_ || noise_->seed != config_.noise_seed || noise_->n_events < n_ev ||
+                noise_->n_per_event != n_tot)) {
+    auto t = std::make_shared<NoiseTable>();
+    t->seed = config_.noise_seed;
+    t->n_events = n_ev;
+    t->n_per_event = n_tot;
+    t->z.resize(n_tot * static_cast<std::size_t>(n_ev));
+    for (int e = 0; e < n_ev; ++e) {
+      std::mt19937 rng(config_.noise_seed + static_cast<std::uint32_t>(e));
+      std::normal_distribution<real> nd(0.0f, 1.0f);
+      real* z = t->z.data() + n_tot * static_cast<std::size_t>(e);
+      for (std::size_t k = 0; k < n_tot; ++k) z[k] = nd(rng);
+    }
+    noise_ = t;
+  }
+  std::vector<real> sc(7 * phantom.scatterers.size());
+  for (std::size_t i = 0; i < phantom.scatterers.size(); ++i) {
+    const Scatterer& p = phantom.scatterers[i];
+    real* d = sc.data() + 7 * i;
+    d[0] = p.position.x;
+    d[1] = p.position.y;
+    d[2] = p.position.z;
+    d[3] = p.amplitude;
+    d[4] = p.velocity.x;
+    d[5] = p.velocity.y;
+    d[6] = p.velocity.z;
+  }
+  usx_sim_desc d{};
+  d.n_elements = n_ch;
+  d.n_samples = n_smp;
+  d.n_events = n_ev;
+  d.sampling_frequency = acq_.sampling_frequency;
+  d.t0 = acq_.t0;
+  d.prf = acq_.prf;
+  d.speed_of_sound = medium_.speed_of_sound;
+  d.wavelength = probe_.wavelength(medium_.speed_of_sound);
+  d.element_width = probe_.element_width;
+  d.oversampling = config_.oversampling;
+  d.element_x = ex.data();
+  d.delays = delays.data();
+  d.apodization = apod.data();
+  d.has_apodization = has_apod.data();
+  d.pulse_n = static_cast<int>(pw.size());
+  d.pulse = pw.data();
+  d.pulse_half = pulse_half;
+  d.attenuation = config_.apply_attenuation ? medium_.attenuation * f0_mhz * 100.0f : 0.0f;
+  d.max_window = config_.max_incident_window_us * 1e-6f;
+  d.trim_amp = db_to_amp(config_.incident_trim_db);
+  d.spreading = config_.apply_spreading;
+  d.directivity = config_.apply_directivity;
+  d.motion = config_.enable_motion;
+  d.full_diffraction = config_.full_transmit_diffraction;
+  d.noise_amp = noise ? db_to_amp(config_.noise_db) : 0.0f;
+  d.noise = noise ? noise_->z.data() : nullptr;
+  usx_ctx* ctx = thread_context().get();
+  check(ctx, usx_simulate(ctx, &d, static_cast<int>(phantom.scatterers.size()), sc.data(), cd.rf.data(),
+                          USX_HOST),
+        "simulate_gpu");
   return cd;
 }
 

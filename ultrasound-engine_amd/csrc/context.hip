@@ -68,6 +68,9 @@ struct usx_ctx {
   size_t ds_n = 0, ds_cap = 0;
   DevBuf tr_h, tr_z, tr_dz, tr_partial, tr_db, tr_loss_part, tr_losses, tr_grads;
   DevBuf mv_w, mv_status, ap_pixels;
+  // simulator
+  Cached s_ex, s_delays, s_firing, s_fw, s_nf, s_pulse, s_noise;
+  DevBuf s_scat, s_live, s_t0, s_n, s_pos, s_wave, s_partial, s_bmax, s_rf;
 };
 
 namespace {
@@ -502,9 +505,11 @@ usx_status usx_ctx_destroy(usx_ctx* c) {
                     &c->ap_master, &c->ap_m, &c->ap_v, &c->ap_wf, &c->ap_wb, &c->ds_x, &c->ds_g,
                     &c->ds_mask, &c->ds_sw, &c->tr_h, &c->tr_z, &c->tr_dz, &c->tr_partial,
                     &c->tr_db, &c->tr_loss_part, &c->tr_losses, &c->tr_grads, &c->mv_w,
-                    &c->mv_status, &c->ap_pixels})
+                    &c->mv_status, &c->ap_pixels, &c->s_scat, &c->s_live, &c->s_t0, &c->s_n,
+                    &c->s_pos, &c->s_wave, &c->s_partial, &c->s_bmax, &c->s_rf})
     if (b->p) (void)hipFree(b->p);
-  for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0})
+  for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0, &c->s_ex,
+                    &c->s_delays, &c->s_firing, &c->s_fw, &c->s_nf, &c->s_pulse, &c->s_noise})
     if (t->dev.p) (void)hipFree(t->dev.p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -883,4 +888,110 @@ usx_status usx_apod_gradient(usx_ctx* c, int n, const uint16_t* x, const float* 
   });
 }
 
-}  // extern "C\"
+usx_status usx_simulate(usx_ctx* c, const usx_sim_desc* d, int n_scat, const float* scatterers,
+                        float* rf, int memory) {
+  return guarded(c, [&] {
+    if (!d) fail(c, USX_ERR_INVALID, "usx_simulate: null descriptor");
+    if (d->n_elements <= 0 || d->n_samples <= 0 || d->n_events < 0)
+      fail(c, USX_ERR_INVALID, "usx_simulate: empty probe or acquisition");
+    if (n_scat < 0 || (n_scat > 0 && !scatterers) || !rf)
+      fail(c, USX_ERR_INVALID, "usx_simulate: null scatterers or output");
+    if (!d->element_x || !d->delays || !d->pulse || d->pulse_n < 1 || d->oversampling < 1)
+      fail(c, USX_ERR_INVALID, "usx_simulate: element, delay and pulse tables are required");
+    if (d->noise_amp > 0.0f && !d->noise)
+      fail(c, USX_ERR_INVALID, "usx_simulate: noise_amp > 0 needs the standard-normal table");
+    const int ne = d->n_elements, nev = d->n_events;
+    usxd::SimArgs S{};
+    S.n_elements = ne;
+    S.n_samples = d->n_samples;
+    S.fs = d->sampling_frequency;
+    S.t0 = d->t0;
+    S.prf = d->prf;
+    S.c = d->speed_of_sound;
+    S.fs_os = d->sampling_frequency * static_cast<float>(d->oversampling);
+    S.wavelength = d->wavelength;
+    S.element_width = d->element_width;
+    S.pulse_n = d->pulse_n;
+    S.pulse_half = d->pulse_half;
+    S.atten = d->attenuation;
+    S.max_window = d->max_window;
+    S.trim_amp = d->trim_amp;
+    S.noise_amp = d->noise_amp;
+    S.spreading = d->spreading;
+    S.directivity = d->directivity;
+    S.motion = d->motion;
+    S.full_diffraction = d->full_diffraction;
+    // incident buffer: ceil((max_window + 2 pulse_half) fs_os) + 2, plus 2 spare
+    S.inc_cap = static_cast<int>(std::ceil((d->max_window + 2.0f * d->pulse_half) * S.fs_os)) + 4;
+    if (S.inc_cap < 8) S.inc_cap = 8;
+    if (usxd::sim_incident_smem(S) > 160 * 1024 || usxd::sim_echo_smem(S) > 160 * 1024)
+      fail(c, USX_ERR_INVALID, "usx_simulate: incident window (%d samples) or record length (%d) too long",
+           S.inc_cap, S.n_samples);
+    // firing lists (simulator.cpp:229-233)
+    std::vector<int> firing(static_cast<size_t>(std::max(nev, 1)) * ne), nfire(std::max(nev, 1));
+    std::vector<float> fw(firing.size());
+    for (int e = 0; e < nev; ++e) {
+      int n = 0;
+      const size_t row = static_cast<size_t>(e) * ne;
+      const bool has = d->has_apodization && d->has_apodization[e] && d->apodization;
+      if (has)
+        for (int i = 0; i < ne; ++i) {
+          const float a = d->apodization[row + i];
+          if (a != 0.0f) {
+            firing[row + n] = i;
+            fw[row + n] = a;
+            ++n;
+          }
+        }
+      if (n == 0)
+        for (int i = 0; i < ne; ++i) {
+          // every element fires: weight 1 for an empty apodization, else its
+          // own (zero) weight, as path_amp reads it
+          firing[row + n] = i;
+          fw[row + n] = has ? d->apodization[row + i] : 1.0f;
+          ++n;
+        }
+      nfire[e] = n;
+    }
+    S.ex = static_cast<const float*>(upload(c, c->s_ex, d->element_x, sizeof(float) * ne));
+    S.delays = static_cast<const float*>(upload(c, c->s_delays, d->delays, sizeof(float) * nev * ne));
+    S.firing = static_cast<const int*>(upload(c, c->s_firing, firing.data(), sizeof(int) * firing.size()));
+    S.fire_w = static_cast<const float*>(upload(c, c->s_fw, fw.data(), sizeof(float) * fw.size()));
+    S.n_firing = static_cast<const int*>(upload(c, c->s_nf, nfire.data(), sizeof(int) * nfire.size()));
+    S.pulse = static_cast<const float*>(upload(c, c->s_pulse, d->pulse, sizeof(float) * d->pulse_n));
+    const size_t n_tot = static_cast<size_t>(ne) * d->n_samples;
+    const float* z = nullptr;
+    if (d->noise_amp > 0.0f)
+      z = static_cast<const float*>(upload(c, c->s_noise, d->noise, sizeof(float) * n_tot * nev));
+    const size_t ns = static_cast<size_t>(std::max(n_scat, 1));
+    float* d_scat = static_cast<float*>(ensure(c, c->s_scat, sizeof(float) * 7 * ns));
+    if (n_scat > 0)
+      hip_check(c, hipMemcpyAsync(d_scat, scatterers, sizeof(float) * 7 * n_scat, hipMemcpyHostToDevice, c->stream),
+                "hipMemcpyAsync(scatterers)");
+    usxd::SimIncident I{};
+    I.live = static_cast<int*>(ensure(c, c->s_live, sizeof(int) * ns));
+    I.t0 = static_cast<float*>(ensure(c, c->s_t0, sizeof(float) * ns));
+    I.n = static_cast<int*>(ensure(c, c->s_n, sizeof(int) * ns));
+    I.pos = static_cast<float4*>(ensure(c, c->s_pos, sizeof(float4) * ns));
+    I.wave = static_cast<float*>(ensure(c, c->s_wave, sizeof(float) * ns * S.inc_cap));
+    const int nc = std::max(1, usxd::sim_chunks(n_scat));
+    float* partial = static_cast<float*>(ensure(c, c->s_partial, sizeof(float) * n_tot * nc));
+    float* bmax = static_cast<float*>(ensure(c, c->s_bmax, sizeof(float) * ((n_tot + 255) / 256 + 1)));
+    float* d_rf = memory == USX_DEVICE
+                      ? rf
+                      : static_cast<float*>(ensure(c, c->s_rf, sizeof(float) * n_tot * std::max(nev, 1)));
+    for (int e = 0; e < nev; ++e)
+      timed(c, "simulate", [&] {
+        usxd::launch_sim_event(S, e, d_scat, n_scat, I, partial, bmax, z ? z + n_tot * e : nullptr,
+                               d_rf + n_tot * e, c->stream);
+      });
+    check_launch(c, "simulate launch");
+    if (memory != USX_DEVICE) {
+      hip_check(c, hipMemcpyAsync(rf, d_rf, sizeof(float) * n_tot * nev, hipMemcpyDeviceToHost, c->stream),
+                "hipMemcpyAsync(rf)");
+      hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+  });
+}
+
+}  // extern "C"

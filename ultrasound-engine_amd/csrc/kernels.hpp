@@ -125,4 +125,39 @@ void launch_apod_gather(int n_list, const int* pixels, int n_ch, const float2* Y
 void launch_apod_step(const ApodNet& N, const ApodData& D, const ApodStep& S, const ApodWork& W,
                       const ApodAdam& opt, float* grads, float* loss_out, hipStream_t st);
 
+// sim.hip: point-scatterer forward model.
+constexpr int kSimChunk = 512;  // scatterers per partial trace
+struct SimArgs {
+  int n_elements, n_samples;
+  float fs, t0, prf, c, fs_os, wavelength, element_width;
+  const float* ex;        // element x, n_elements
+  const float* delays;    // [event][element] firing delays
+  const int* firing;      // [event][element] firing element indices (first n_firing[event])
+  const float* fire_w;    // [event][element] their apodization weights
+  const int* n_firing;    // [event]
+  const float* pulse;
+  int pulse_n;
+  float pulse_half;
+  float atten;            // dB per metre at the centre frequency, 0 = off
+  float max_window;       // s
+  float trim_amp;
+  float noise_amp;        // 0 = no receive noise
+  int inc_cap;            // longest incident waveform (oversampled samples)
+  int spreading, directivity, motion, full_diffraction;
+};
+struct SimIncident {
+  int* live;
+  float* t0;
+  int* n;
+  float4* pos;            // position after motion, amplitude
+  float* wave;            // [scatterer][inc_cap]
+};
+size_t sim_incident_smem(const SimArgs& S);
+size_t sim_echo_smem(const SimArgs& S);
+int sim_chunks(int n_scat);
+// One event: rf (n_elements * n_samples) of event ev; z the event's
+// standard normals (nullptr: no noise); blockmax: ceil(n_el n_s / 256) floats.
+void launch_sim_event(const SimArgs& S, int ev, const float* scat, int n_scat, const SimIncident& I,
+                      float* partial, float* blockmax, const float* z, float* rf, hipStream_t st);
+
 }  // namespace usxd
