@@ -10,8 +10,10 @@ Workload (one step = one frame through the hot path, RF already in HBM):
     beamforming (subarray 64, diagonal loading 1e-2, +-2 depth averaging,
     f/1.5) onto a 256 x 640 grid; complex64 frame out.
 The headline `value` is MV frames/s. DAS, DAS + coherence factor and DMAS
-frame rates on the same RF, and the end-to-end (host RF in, host frame out)
-MV rate, are reported under "extras".
+frame rates on the same RF, the end-to-end (host RF in, host frame out)
+MV rate, and the learned-apodization rate (trained network shipped in
+usx/data, with its gCNR on the cyst beside MV's and its BF16 MFMA roofline)
+are reported under "extras".
 
 roofline: the dominant kernel of the MV step (`mv`, covariance + solve) is
 priced against the FP32 MFMA peak with the reference's FLOP count
@@ -41,6 +43,7 @@ sys.path.insert(0, os.path.join(ROOT, "ultrasound-engine_amd"))
 
 METRIC = "frames/s @256x640x11PW: DAS, MV, learned-MV at matched gCNR; % HBM roofline"
 PEAK_FP32_MFMA = 157.3e12
+PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 
 
@@ -100,10 +103,12 @@ def mv_reference_flops(probe, grid, f_number=1.5, subarray=64, range_avg=2):
     return float(np.sum(np.where(live, cov + sol, 0.0)))
 
 
-def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False):
+def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False, apod=None):
     """Device-resident frames/s: RF in HBM before the timed region; the
     timed region is bracketed by synchronizations on both sides."""
     runner = usx.DeviceRunner(grid, pc, raw)
+    if apod is not None:
+        runner.set_apodization(apod)
     rf = torch.from_numpy(np.ascontiguousarray(np.asarray(raw.rf))).to("cuda:0")
     frame = torch.empty((grid.n_axis0, grid.n_axis1), dtype=torch.complex64, device="cuda:0")
     torch.cuda.synchronize()
@@ -199,6 +204,29 @@ def main():
         extras[f"{name}_kernels_ms"] = {n: v[1] / v[0] for n, v in k.items()}
         log(f"  {name}: {ms:.4f} ms/frame, kernels {k}")
     extras["mv_end_to_end_fps"] = 1e3 / time_host(usx, raw, grid, pc_mv, max(3, a.steps // 4))
+
+    # learned apodization (trained checkpoint shipped with the package)
+    from usx import learned as L
+    apod, _ = L.load_checkpoint(os.path.join(ROOT, "ultrasound-engine_amd", "usx", "data",
+                                             "apod_mlp.npz"))
+    pc_l = pipeline_config(usx, "Learned")
+    l_ms, _, lk, l_frame = time_device(usx, torch, raw, grid, pc_l, max(a.steps, 50), a.warmup,
+                                       profile=True, apod=apod)
+    extras["learned_fps"] = 1e3 / l_ms
+    extras["learned_speedup_vs_mv"] = mv_ms / l_ms
+    extras["learned_kernels_ms"] = {n: v[1] / v[0] for n, v in lk.items()}
+    cyst = ((0.0, 0.020), 0.003)
+    extras["gcnr_mv"] = usx.metrics.gcnr(np.abs(mv_frame), grid, *cyst)
+    extras["gcnr_learned"] = usx.metrics.gcnr(np.abs(l_frame), grid, *cyst)
+    extras["learned_iq_err_vs_mv"] = float(np.linalg.norm(l_frame - mv_frame) / np.linalg.norm(mv_frame))
+    infer_ms = extras["learned_kernels_ms"]["apod_infer"]
+    infer_flops = 2.0 * grid.n_pixels * 4 * 256 * 256
+    extras["apod_infer_roofline"] = {"bound": "mfma", "achieved": infer_flops / (infer_ms * 1e-3) / 1e12,
+                                     "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
+                                     "frac": infer_flops / (infer_ms * 1e-3) / PEAK_BF16_MFMA,
+                                     "flops_per_launch": infer_flops, "kernel_ms": infer_ms}
+    log(f"  learned: {l_ms:.4f} ms/frame ({mv_ms / l_ms:.1f}x MV), kernels {lk}, "
+        f"gCNR {extras['gcnr_learned']:.3f} vs MV {extras['gcnr_mv']:.3f}")
     extras["mv_kernels_ms"] = {n: v[1] / v[0] for n, v in kern.items()}
 
     flops = mv_reference_flops(probe, grid)
