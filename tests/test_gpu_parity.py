@@ -311,3 +311,37 @@ def test_mv_precision_variants_against_fp64(usx_mod, port, O, config0):
         errs[(prec, refine)] = rel_l2(got, ref64)
         assert errs[(prec, refine)] < bound, (prec, refine, errs)
     assert errs[("FP16", True)] < errs[("BF16", True)]
+
+
+def test_mv_192_channels_subarray_96(usx_mod, port, O):
+    """192-element probe, subarray 96 at f/1: L reaches 96, so the 80- and
+    96-row buckets (two matrix rows per lane) run beside the smaller ones."""
+    case = make_case(usx_mod, O, n_el=192, n_samples=2048, n_tx=3, n_scat=2500,
+                     grid=(12, 0.01, 0.07, 120), box=(-0.028, 0.028, 0.005, 0.072))
+    d = usx_mod.DemodConfig()
+    d.decimation = 4
+    iq = usx_mod.demodulate(case.raw, d)
+    s, txs = oracle_inputs(O, case)
+    s2 = O.Setup(**{**s.__dict__, "sampling_frequency": iq.acq.sampling_frequency,
+                    "n_samples": iq.n_samples})
+    bfc = bf_config(usx_mod, "MV", subarray=96, f_number=1.0)
+    got = np.asarray(usx_mod.Beamformer(bfc).beamform(iq, case.grid).data)
+    ref, ex = port.beamform(s2, txs, np.asarray(iq.iq), case.gspec, O.Bf.of(bfc),
+                            fdem=iq.demodulation_frequency, info=True)
+    L = ex["info"][:, 1]
+    assert L.max() == 96 and (L > 64).sum() > 50 and (L[L > 0] <= 16).any()
+    assert rel_l2(got, ref) < TOL_MV, rel_l2(got, ref)
+    # the same with the effective weights on (the label path)
+    run = usx_mod.DeviceRunner(case.grid, _pc(usx_mod, bfc), case.raw)
+    frame, w, st = run.mv_weights(case.raw)
+    assert rel_l2(frame, ref) < TOL_MV
+    _, exw = port.beamform(s2, txs, np.asarray(iq.iq), case.gspec, O.Bf.of(bfc),
+                           fdem=iq.demodulation_frequency, weights=True)
+    assert rel_l2(w.reshape(-1, 192), exw["weights"]) < TOL_MV
+
+
+def _pc(usx, bfc):
+    pc = usx.PipelineConfig()
+    pc.demod.decimation = 4
+    pc.beamformer = bfc
+    return pc

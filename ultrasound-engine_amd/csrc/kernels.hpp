@@ -61,6 +61,7 @@ struct MvArgs {
   // active channel.
   float2* weights;
   int* status;
+  int occupancy;   // persistent workgroups per CU per bucket (0: as many as LDS allows, <= 32)
 };
 // counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
@@ -68,7 +69,7 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
 // Largest covariance size bucket (L rounded up to 16) a configuration can produce.
 int mv_max_bucket_p(int n_ch, int subarray);
 size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P);
-constexpr int kMvMaxP = 64;
+constexpr int kMvMaxP = 96;
 
 // apod.hip: learned apodization MLP (256 -> 3 x 256 GELU -> 256, bf16 MFMA).
 constexpr int kApodFeatures = 256;

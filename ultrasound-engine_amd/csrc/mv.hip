@@ -70,6 +70,16 @@ __device__ __forceinline__ float quant(float x) {
 
 constexpr int kMaxBuckets = 16;  // P up to 256
 
+// Element q of a per-lane slot array with q uniform across the wavefront
+// (a select, so the array stays in registers).
+template <int NS>
+__device__ __forceinline__ float pick(const float (&v)[NS], int q) {
+  float r = v[0];
+#pragma unroll
+  for (int k = 1; k < NS; ++k) r = q == k ? v[k] : r;
+  return r;
+}
+
 // Coherence factor |sum y|^2 / (n sum |y|^2) over the active centre vector.
 __device__ __forceinline__ float coherence(const float2* yc, const int* act, int na, int lane) {
   float cr = 0.0f, ci = 0.0f, inc = 0.0f;
@@ -186,6 +196,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
                                                const int* __restrict__ counts,
                                                const int* __restrict__ lists, int na_max) {
   constexpr int NB = P / 16;
+  constexpr int NS = (P + 63) / 64;  // matrix rows (and vector elements) per lane
   constexpr int NPAIRS = NB * (NB + 1) / 2;
   constexpr int BUCKET = P / 16 - 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -350,36 +361,47 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
     }
     __syncthreads();
 
-    // ---- 5. Cholesky, left-looking by columns: lane i owns row i -------------
+    // ---- 5. Cholesky, left-looking by columns: lane i owns rows i, i + 64 ----
     //   L_ij = (A_ij - sum_{k<j} L_ik conj(L_jk)) / L_jj, L_jj = sqrt(A_jj - sum |L_jk|^2),
     // the reference's sums (beamformer.cpp:48-63); one barrier per column.
     bool ok = true;
     for (int j = 0; j < L; ++j) {
-      float sr = 0.0f, si = 0.0f;
-      if (lane >= j && lane < L) {
-        const float2* ri_ = A + lane * S;
-        const float2* rj = A + j * S;
-        sr = ri_[j].x;
-        si = ri_[j].y;
+      float sr[NS], si[NS];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const int row = lane + 64 * q;
+        sr[q] = 0.0f;
+        si[q] = 0.0f;
+        if (row >= j && row < L) {
+          const float2* ri_ = A + row * S;
+          const float2* rj = A + j * S;
+          float ar = ri_[j].x, ai = ri_[j].y;
 #pragma unroll 4
-        for (int k = 0; k < j; ++k) {
-          const float2 a = ri_[k], b = rj[k];  // -= a * conj(b)
-          sr = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr));
-          si = fmaf(-a.y, b.x, fmaf(a.x, b.y, si));
+          for (int k = 0; k < j; ++k) {
+            const float2 a = ri_[k], b = rj[k];  // -= a * conj(b)
+            ar = fmaf(-a.x, b.x, fmaf(-a.y, b.y, ar));
+            ai = fmaf(-a.y, b.x, fmaf(a.x, b.y, ai));
+          }
+          sr[q] = ar;
+          si[q] = ai;
         }
       }
-      const float d = lane_bcast(sr, j);
+      const float d = lane_bcast(pick<NS>(sr, j >> 6), j & 63);
       if (!(d > 0.0f)) {
         ok = false;
         break;
       }
       const float ljj = sqrtf(d);
       const float rj = 1.0f / ljj;
-      if (lane == j) {
-        A[j * S + j] = make_float2(ljj, 0.0f);
-        dg[j].y = rj;
-      } else if (lane > j && lane < L) {
-        A[lane * S + j] = make_float2(sr * rj, si * rj);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const int row = lane + 64 * q;
+        if (row == j) {
+          A[j * S + j] = make_float2(ljj, 0.0f);
+          dg[j].y = rj;
+        } else if (row > j && row < L) {
+          A[row * S + j] = make_float2(sr[q] * rj, si[q] * rj);
+        }
       }
       __syncthreads();
     }
@@ -393,76 +415,109 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       }
       out = make_float2(wsum(sr), wsum(si));
     } else {
-      // ---- 6. solves with the vector in registers (lane i holds b_i) --------
-      auto solve = [&](float br, float bi) {
+      // ---- 6. solves with the vector in registers (element i + 64 q in slot q
+      // of lane i) and the pivot element broadcast by readlane: no barriers ----
+      float wr[NS], wi[NS];
+      auto solve = [&](float (&br)[NS], float (&bi)[NS]) {
         for (int k = 0; k < L; ++k) {  // L y = b
           const float rk = dg[k].y;
-          const float yr = lane_bcast(br, k) * rk, yi = lane_bcast(bi, k) * rk;
-          if (lane == k) {
-            br = yr;
-            bi = yi;
-          } else if (lane > k && lane < L) {
-            const float2 l = A[lane * S + k];
-            br = fmaf(-l.x, yr, fmaf(l.y, yi, br));
-            bi = fmaf(-l.x, yi, fmaf(-l.y, yr, bi));
+          const float yr = lane_bcast(pick<NS>(br, k >> 6), k & 63) * rk;
+          const float yi = lane_bcast(pick<NS>(bi, k >> 6), k & 63) * rk;
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const int row = lane + 64 * q;
+            if (row == k) {
+              br[q] = yr;
+              bi[q] = yi;
+            } else if (row > k && row < L) {
+              const float2 l = A[row * S + k];
+              br[q] = fmaf(-l.x, yr, fmaf(l.y, yi, br[q]));
+              bi[q] = fmaf(-l.x, yi, fmaf(-l.y, yr, bi[q]));
+            }
           }
         }
         for (int k = L - 1; k >= 0; --k) {  // L^H x = y
           const float rk = dg[k].y;
-          const float xr = lane_bcast(br, k) * rk, xi = lane_bcast(bi, k) * rk;
-          if (lane == k) {
-            br = xr;
-            bi = xi;
-          } else if (lane < k) {
-            const float2 l = A[k * S + lane];  // -= conj(L[k][lane]) * x_k
-            br = fmaf(-l.x, xr, fmaf(-l.y, xi, br));
-            bi = fmaf(-l.x, xi, fmaf(l.y, xr, bi));
+          const float xr = lane_bcast(pick<NS>(br, k >> 6), k & 63) * rk;
+          const float xi = lane_bcast(pick<NS>(bi, k >> 6), k & 63) * rk;
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const int row = lane + 64 * q;
+            if (row == k) {
+              br[q] = xr;
+              bi[q] = xi;
+            } else if (row < k) {
+              const float2 l = A[k * S + row];  // -= conj(L[k][row]) * x_k
+              br[q] = fmaf(-l.x, xr, fmaf(-l.y, xi, br[q]));
+              bi[q] = fmaf(-l.x, xi, fmaf(l.y, xr, bi[q]));
+            }
           }
         }
-        return make_float2(br, bi);
       };
-      float2 w = solve(lane < L ? 1.0f : 0.0f, 0.0f);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        wr[q] = lane + 64 * q < L ? 1.0f : 0.0f;
+        wi[q] = 0.0f;
+      }
+      solve(wr, wi);
 
       if (M.refine) {  // ---- 7. one refinement step, FP64 residual ----------
-        if (lane < P) rv[lane] = lane < L ? w : make_float2(0.0f, 0.0f);
-        __syncthreads();
-        float2 r = make_float2(0.0f, 0.0f);
-        if (lane < L) {
-          const int a = lane;
-          double rr = 1.0 - static_cast<double>(dg[a].x) * w.x;
-          double ri = -static_cast<double>(dg[a].x) * w.y;
-          for (int b = 0; b < L; ++b) {
-            if (b == a) continue;
-            float2 rab = (a < b) ? A[a * S + b] : A[b * S + a];
-            if (a > b) rab.y = -rab.y;
-            const float2 wb = rv[b];
-            rr = fma(-static_cast<double>(rab.x), wb.x, fma(static_cast<double>(rab.y), wb.y, rr));
-            ri = fma(-static_cast<double>(rab.x), wb.y, fma(-static_cast<double>(rab.y), wb.x, ri));
-          }
-          r = make_float2(static_cast<float>(rr), static_cast<float>(ri));
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const int row = lane + 64 * q;
+          if (row < P) rv[row] = row < L ? make_float2(wr[q], wi[q]) : make_float2(0.0f, 0.0f);
         }
-        const float2 dlt = solve(r.x, r.y);
-        if (lane < L) {
-          w.x += dlt.x;
-          w.y += dlt.y;
+        __syncthreads();
+        float rr_[NS], ri_[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const int a = lane + 64 * q;
+          rr_[q] = 0.0f;
+          ri_[q] = 0.0f;
+          if (a < L) {
+            double rr = 1.0 - static_cast<double>(dg[a].x) * wr[q];
+            double ri = -static_cast<double>(dg[a].x) * wi[q];
+            for (int b = 0; b < L; ++b) {
+              if (b == a) continue;
+              float2 rab = (a < b) ? A[a * S + b] : A[b * S + a];
+              if (a > b) rab.y = -rab.y;
+              const float2 wb = rv[b];
+              rr = fma(-static_cast<double>(rab.x), wb.x, fma(static_cast<double>(rab.y), wb.y, rr));
+              ri = fma(-static_cast<double>(rab.x), wb.y, fma(-static_cast<double>(rab.y), wb.x, ri));
+            }
+            rr_[q] = static_cast<float>(rr);
+            ri_[q] = static_cast<float>(ri);
+          }
+        }
+        solve(rr_, ri_);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          if (lane + 64 * q < L) {
+            wr[q] += rr_[q];
+            wi[q] += ri_[q];
+          }
         }
       }
 
       // ---- 8. weights on the subarray-averaged centre snapshot --------------
       float dr = 0.0f, di = 0.0f, orr = 0.0f, oi = 0.0f;
-      if (lane < L) {
-        const float2* cen = rows + crow * na;
-        float sr = 0.0f, si = 0.0f;
-        for (int s0 = 0; s0 < ns; ++s0) {
-          sr += cen[s0 + lane].x;
-          si += cen[s0 + lane].y;
+      const float2* cen = rows + crow * na;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const int a = lane + 64 * q;
+        if (a < L) {
+          float sr = 0.0f, si = 0.0f;
+          for (int s0 = 0; s0 < ns; ++s0) {
+            sr += cen[s0 + a].x;
+            si += cen[s0 + a].y;
+          }
+          sr /= static_cast<float>(ns);
+          si /= static_cast<float>(ns);
+          dr += wr[q];
+          di += wi[q];
+          orr += wr[q] * sr + wi[q] * si;  // conj(w) * avg
+          oi += wr[q] * si - wi[q] * sr;
         }
-        sr /= static_cast<float>(ns);
-        si /= static_cast<float>(ns);
-        dr = w.x;
-        di = w.y;
-        orr = w.x * sr + w.y * si;  // conj(w) * avg
-        oi = w.x * si - w.y * sr;
       }
       dr = wsum(dr);
       di = wsum(di);
@@ -484,17 +539,27 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
             static_cast<float>((static_cast<double>(oi) * dr - static_cast<double>(orr) * di) / dd) * nf);
         if (M.weights) {
           // g_c = n / (n_sub sum w) * sum_{a = max(0, c - n_sub + 1)}^{min(L - 1, c)} conj(w_a):
-          // inclusive prefix sums of conj(w) (lane a < L) through a wave scan.
-          float pr = lane < L ? w.x : 0.0f, pi = lane < L ? -w.y : 0.0f;
+          // inclusive prefix sums of conj(w) over the L entries (a wave scan per
+          // slot, slot q offset by the total of the slots before it).
+          float carry_r = 0.0f, carry_i = 0.0f;
 #pragma unroll
-          for (int o = 1; o < 64; o <<= 1) {
-            const float qr = __shfl_up(pr, o, 64), qi = __shfl_up(pi, o, 64);
-            if (lane >= o) {
-              pr += qr;
-              pi += qi;
+          for (int q = 0; q < NS; ++q) {
+            const int a = lane + 64 * q;
+            float pr = a < L ? wr[q] : 0.0f, pi = a < L ? -wi[q] : 0.0f;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+              const float qr = __shfl_up(pr, o, 64), qi = __shfl_up(pi, o, 64);
+              if (lane >= o) {
+                pr += qr;
+                pi += qi;
+              }
             }
+            pr += carry_r;
+            pi += carry_i;
+            if (a < P) rv[a] = make_float2(pr, pi);
+            carry_r = lane_bcast(pr, 63);
+            carry_i = lane_bcast(pi, 63);
           }
-          if (lane < P) rv[lane] = make_float2(pr, pi);
           __syncthreads();
           const double sc = static_cast<double>(na) / (static_cast<double>(ns) * dd);
           float2* gw = M.weights + static_cast<size_t>(pix) * n_ch;
@@ -533,11 +598,20 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
   const size_t smem = mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
   const int n_pix = M.n0 * M.n1;
-  const int per_cu = std::max(1, std::min(32, static_cast<int>(160 * 1024 / smem)));
+  // persistent grid: as many single-wavefront workgroups per CU as LDS allows
+  // (capped at 32, or at the tuned occupancy)
+  const int cap = M.occupancy > 0 ? M.occupancy : 32;
+  const int per_cu = std::max(1, std::min(cap, static_cast<int>(160 * 1024 / smem)));
   const int grid = std::min(n_pix, 256 * per_cu);
-#define USX_MV(PR)                                                                       \
-  hipLaunchKernelGGL((mv_solve<P, PR>), dim3(grid), dim3(64), smem, st, M, Y, mask, frame, \
-                     acc, counts, lists, na_max)
+#define USX_MV(PR)                                                                            \
+  do {                                                                                        \
+    static const hipError_t attr = hipFuncSetAttribute(                                       \
+        reinterpret_cast<const void*>(mv_solve<P, PR>),                                       \
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                              \
+    (void)attr;                                                                               \
+    hipLaunchKernelGGL((mv_solve<P, PR>), dim3(grid), dim3(64), smem, st, M, Y, mask, frame,  \
+                       acc, counts, lists, na_max);                                           \
+  } while (0)
   if (M.precision == 1) USX_MV(1);
   else if (M.precision == 2) USX_MV(2);
   else USX_MV(0);
@@ -560,6 +634,8 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
   if (pmax >= 32) launch_bucket<32>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 48) launch_bucket<48>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 64) launch_bucket<64>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 80) launch_bucket<80>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 96) launch_bucket<96>(M, Y, mask, frame, acc, counts, lists, st);
 }
 
 }  // namespace usxd
