@@ -11,9 +11,10 @@ Workload (one step = one frame through the hot path, RF already in HBM):
     f/1.5) onto a 256 x 640 grid; complex64 frame out.
 The headline `value` is MV frames/s. DAS, DAS + coherence factor and DMAS
 frame rates on the same RF, the end-to-end (host RF in, host frame out)
-MV rate, and the learned-apodization rate (trained network shipped in
+MV rate, the learned-apodization rate (trained network shipped in
 usx/data, with its gCNR on the cyst beside MV's and its BF16 MFMA roofline)
-are reported under "extras".
+and the uneven batched stream (64 frames of seed 1, DAS + MV each, one HIP
+graph) are reported under "extras".
 
 roofline: the dominant kernel of the MV step (`mv`, covariance + solve) is
 priced against the FP32 MFMA peak with the reference's FLOP count
@@ -170,12 +171,43 @@ def cpu_baseline(usx, probe, med, acq, txs, raw, n_lines=128):
             "sample_seconds": round(sample_s, 3)}
 
 
+def run_stream(n_frames, cache_path=None):
+    """The uneven batched stream (seed 1): n frames, each beamformed with DAS
+    and MV (subarray = elements / 2), recorded into one HIP graph; launch
+    shapes from the autotuner's versioned cache (tuned where missing)."""
+    from usx import stream as S
+    specs = S.draw(seed=1, n=n_frames)
+    cache = S.load_cache(cache_path or S.CACHE_PATH)
+    t = time.perf_counter()
+    st = S.Stream(specs, tune=True, cache=cache, log=log)
+    setup_s = time.perf_counter() - t
+    if st.tuned:
+        S.save_cache(st.cache, cache_path or S.CACHE_PATH)
+    st.run(graph=True)
+    eager_ms = st.run(graph=False, reps=2)
+    graph_ms = st.run(graph=True, reps=3)
+    pixels = sum(sp["grid"][0] * sp["grid"][1] for sp in specs)
+    res = {"frames": n_frames, "jobs": 2 * n_frames, "graph_ms": graph_ms, "eager_ms": eager_ms,
+           "frames_per_s": n_frames / (graph_ms * 1e-3), "mpixels": pixels / 1e6,
+           "elements": sorted({sp["elements"] for sp in specs}),
+           "transmits": [min(len(sp["angles_deg"]) for sp in specs),
+                         max(len(sp["angles_deg"]) for sp in specs)],
+           "setup_s": setup_s, "classes_tuned_now": len(st.tuned),
+           "classes": len({S.shape_class(sp, c) for sp in specs for c in ("DAS", "MV")})}
+    log(f"  stream: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description="MI355X imaging-path benchmark")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--skip-stream", action="store_true", help="omit the uneven-stream leg")
+    ap.add_argument("--stream-frames", type=int, default=64)
+    ap.add_argument("--tile-cache", default=None,
+                    help="autotuner cache file (default: the package's usx/data/tile_cache.json)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the imaging path is "
@@ -239,6 +271,9 @@ def main():
     das_bytes = raw.rf.nbytes + grid.n_pixels * 8
     das_total_ms = sum(extras["das_kernels_ms"].values())
     extras["das_hbm_frac"] = das_bytes / (das_total_ms * 1e-3) / PEAK_HBM
+
+    if not a.skip_stream:
+        extras["stream"] = run_stream(a.stream_frames, a.tile_cache)
 
     cpu = None
     if not a.skip_cpu:

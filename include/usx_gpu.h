@@ -164,6 +164,10 @@ typedef struct {
   /* additions with no reference counterpart (default 0 = reference behaviour) */
   int mv_precision;        /* 0 FP32 covariance, 1 BF16 inputs, 2 FP16 inputs (FP32 accumulate) */
   int mv_refine;           /* 1: one step of iterative refinement, FP64 residual */
+  /* launch-shape tunables (no reference counterpart; 0 = default), as
+   * usx_autotune picks them */
+  int tile_slots;          /* transmit events per beamforming launch, 1..16 (default 16) */
+  int tile_mv_occupancy;   /* persistent MV workgroups per CU (default: as LDS allows) */
 } usx_bf_desc;
 
 /* data: n_events * n_channels * n_samples (float RF or usx_c32 IQ);
@@ -281,6 +285,40 @@ typedef struct {
  * to float rounding (device powf/sinf in the amplitudes), deterministically. */
 usx_status usx_simulate(usx_ctx* ctx, const usx_sim_desc* s, int n_scatterers,
                         const float* scatterers, float* rf, int memory);
+
+/* ---- device buffers (no reference counterpart) --------------------------
+ * Plain device memory for callers without a device allocator of their own
+ * (the frame-sequence binding). Copies are synchronous; usx_buffer_zero is a
+ * fill kernel on the context stream, so it may be captured in a graph. */
+usx_status usx_buffer_alloc(usx_ctx* ctx, size_t bytes, void** dev);
+usx_status usx_buffer_free(usx_ctx* ctx, void* dev);
+usx_status usx_buffer_upload(usx_ctx* ctx, void* dev, const void* host, size_t bytes);
+usx_status usx_buffer_download(usx_ctx* ctx, void* host, const void* dev, size_t bytes);
+usx_status usx_buffer_zero(usx_ctx* ctx, void* dev, size_t bytes);
+
+/* ---- frame sequences and launch-shape tuning (no reference counterpart) --- */
+
+/* A fixed sequence of n frames, frame f: RF (device, rf[f]) -> demodulate
+ * (d[f]) -> beamform (b[f]) -> frame (device, frames[f]), each with its own
+ * probe, grid, transmit count and configuration (the uneven batched stream).
+ * Creation uploads every frame's descriptor tables once, sizes the scratch
+ * for the largest frame, runs the sequence once and captures it into one
+ * HIP graph. The pointers are bound at creation. */
+typedef struct usx_plan usx_plan;
+usx_status usx_plan_create(usx_ctx* ctx, int n, const usx_demod_desc* d, const usx_bf_desc* b,
+                           const float* const* rf, usx_c32* const* frames, usx_plan** out);
+/* Enqueue the sequence on the context stream: the graph (use_graph = 1) or
+ * the same kernels launched one by one (0). Asynchronous. */
+usx_status usx_plan_launch(usx_ctx* ctx, usx_plan* plan, int use_graph);
+usx_status usx_plan_destroy(usx_ctx* ctx, usx_plan* plan);
+
+/* Time the candidate launch shapes of one frame (RF and frame in device
+ * memory; transmit events per beamforming launch 4/8/16, and for MV the
+ * persistent occupancy 0/4/8/16 workgroups per CU), each over `reps`
+ * repetitions after a warm-up, and write the fastest into b->tile_slots and
+ * b->tile_mv_occupancy (best_ms, may be NULL: its time per frame). */
+usx_status usx_autotune(usx_ctx* ctx, const usx_demod_desc* d, usx_bf_desc* b, const float* rf,
+                        usx_c32* frame, int reps, float* best_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,93 @@
+"""The uneven batched stream: one captured HIP graph over frames of different
+probe, grid and transmit count (usx.FrameSequence, usx.stream).
+
+* The graph replay equals the same kernels launched one by one, bit for bit,
+  on every replay, with other device work and synchronizations between the
+  replays.
+* Every frame equals ImagingPipeline.process_frame with the same
+  configuration (and launch shape), bit for bit.
+* Frames of the stream match the oracle (DAS 1e-5, MV 1e-4).
+* The autotuner's choices change results by float rounding at most.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_stream(usx_mod):
+    from usx import stream as S
+    specs = S.draw(seed=1, n=6, max_scale=1.4, max_samples=1400)
+    cache = {"version": S.CACHE_VERSION, "target": "gfx950", "entries": {}}
+    return S.Stream(specs, tune=True, cache=cache)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def test_graph_replays_equal_eager_launches(small_stream):
+    st = small_stream
+    st.run(graph=False)
+    eager = st.frames()
+    assert all(np.isfinite(f.view(np.float32)).all() for f in eager)
+    assert any(np.abs(f).max() > 0 for f in eager)
+    for _ in range(3):
+        st.seq.clear()
+        st.seq.synchronize()
+        assert not any(np.abs(f).max() > 0 for f in st.frames())
+        st.run(graph=True)
+        for a, b in zip(st.frames(), eager):
+            assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_stream_frames_equal_the_pipeline(usx_mod, small_stream):
+    st = small_stream
+    st.run(graph=True)
+    got = st.frames()
+    for raw, grid, pc, f in zip(st.raws, st.grids, st.cfgs, got):
+        ref = np.asarray(usx_mod.ImagingPipeline(grid, pc).process_frame(raw).data)
+        assert np.array_equal(_bits(ref), _bits(f))
+
+
+def test_stream_frames_match_the_oracle(usx_mod, port, O, small_stream):
+    st = small_stream
+    st.run(graph=True)
+    got = st.frames()
+    # the first frame of the stream, DAS and MV
+    raw, grid = st.raws[0], st.grids[0]
+    s = O.Setup.of(raw.probe, raw.medium, raw.acq)
+    txs = [O.Tx.of(t) for t in raw.transmits]
+    gs = O.GridSpec(0, *st.extents[0])
+    for k, tol in ((0, 1e-5), (1, 1e-4)):
+        ref, _ = port.process_frame(s, txs, np.asarray(raw.rf), gs, O.Demod(decimation=4),
+                                    O.Bf.of(st.cfgs[k].beamformer))
+        assert rel_l2(got[k], ref) < tol, (k, rel_l2(got[k], ref))
+
+
+def test_autotuned_shapes_keep_the_results(usx_mod, small_stream):
+    """The MV occupancy only reorders independent pixels (bit-identical
+    frames); the event-slot count regroups the sums over transmits (DAS, and
+    MV's channel vectors), which moves results by float rounding only."""
+    st = small_stream
+    assert st.tuned, "the autotuner ran on every shape class"
+    for key, ent in st.tuned.items():
+        assert ent["slots"] in (4, 8, 16) and ent["mv_occupancy"] in (0, 4, 8, 16) and ent["ms"] > 0
+    for raw, grid, pc in zip(st.raws, st.grids, st.cfgs):
+        tuned = np.asarray(usx_mod.ImagingPipeline(grid, pc).process_frame(raw).data)
+        b = pc.beamformer
+        slots = b.tile_slots
+        b.tile_mv_occupancy = 0
+        pc.beamformer = b
+        same_slots = np.asarray(usx_mod.ImagingPipeline(grid, pc).process_frame(raw).data)
+        assert np.array_equal(_bits(tuned), _bits(same_slots))
+        b.tile_slots = 0
+        pc.beamformer = b
+        plain = np.asarray(usx_mod.ImagingPipeline(grid, pc).process_frame(raw).data)
+        assert rel_l2(tuned, plain) < 1e-5, (slots, rel_l2(tuned, plain))

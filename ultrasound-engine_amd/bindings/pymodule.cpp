@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <algorithm>
 
@@ -130,6 +131,51 @@ class DeviceRunner {
   bool is_rf() const { return is_rf_; }
   int iq_samples() const { return bplan_.desc.n_samples; }
   int frame_pixels() const { return bplan_.desc.n_axis0 * bplan_.desc.n_axis1; }
+
+  // Pick the launch shape of this runner's frame by timing the candidates
+  // (usx_autotune) on the given device buffers; returns (slots, mv
+  // occupancy, ms per frame) and keeps the choice for run().
+  py::tuple autotune(std::uintptr_t in_ptr, std::uintptr_t frame_ptr, int reps) {
+    if (!is_rf_) throw std::invalid_argument("autotune: the runner's template must be RF");
+    usx_ctx* c = ctx_.get();
+    float ms = 0.0f;
+    {
+      py::gil_scoped_release release;
+      check(c, usx_autotune(c, &dplan_.desc, &bplan_.desc, reinterpret_cast<const float*>(in_ptr),
+                            reinterpret_cast<usx_c32*>(frame_ptr), reps, &ms),
+            "autotune");
+    }
+    return py::make_tuple(bplan_.desc.tile_slots, bplan_.desc.tile_mv_occupancy, ms);
+  }
+
+  // The same on host channel data of the template's shape, uploaded into
+  // temporary device buffers.
+  py::tuple autotune_host(const ChannelData& data, int reps) {
+    if (!is_rf_) throw std::invalid_argument("autotune: the runner's template must be RF");
+    if (data.rf.size() != static_cast<std::size_t>(dplan_.desc.n_traces) * dplan_.desc.n_in)
+      throw std::invalid_argument("channel data does not match the runner's template");
+    usx_ctx* c = ctx_.get();
+    void* rf = nullptr;
+    void* fr = nullptr;
+    const std::size_t nf = static_cast<std::size_t>(bplan_.desc.n_axis0) * bplan_.desc.n_axis1;
+    check(c, usx_buffer_alloc(c, data.rf.size() * sizeof(float), &rf), "buffer_alloc");
+    const usx_status st = usx_buffer_alloc(c, nf * sizeof(usx_c32), &fr);
+    if (st != USX_OK) {
+      usx_buffer_free(c, rf);
+      check(c, st, "buffer_alloc");
+    }
+    try {
+      check(c, usx_buffer_upload(c, rf, data.rf.data(), data.rf.size() * sizeof(float)), "buffer_upload");
+      py::tuple r = autotune(reinterpret_cast<std::uintptr_t>(rf), reinterpret_cast<std::uintptr_t>(fr), reps);
+      usx_buffer_free(c, rf);
+      usx_buffer_free(c, fr);
+      return r;
+    } catch (...) {
+      usx_buffer_free(c, rf);
+      usx_buffer_free(c, fr);
+      throw;
+    }
+  }
 
   // ---- learned apodization (no reference counterpart) ----------------------
   using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
@@ -261,6 +307,140 @@ class DeviceRunner {
   DemodPlan dplan_;
   BeamformPlan bplan_;
   GpuContext ctx_;
+};
+
+// A fixed sequence of RF frames, each with its own probe, grid, transmits and
+// configuration, recorded once into a HIP graph (usx_plan_create) and
+// replayed. The sequence owns the device memory: the RF of every distinct
+// ChannelData (frames that share one are beamformed from one copy) and one
+// output frame per entry.
+class FrameSequence {
+ public:
+  FrameSequence(const std::vector<ChannelData*>& likes, const std::vector<ScanGrid*>& grids,
+                const std::vector<PipelineConfig>& cfgs) {
+    const std::size_t n = likes.size();
+    if (n == 0 || grids.size() != n || cfgs.size() != n)
+      throw std::invalid_argument("FrameSequence: likes, grids and configs must have one entry per frame");
+    usx_ctx* c = ctx_.get();
+    dplans_.resize(n);
+    bplans_.resize(n);
+    std::vector<const float*> rf(n);
+    std::vector<usx_c32*> fr(n);
+    for (std::size_t f = 0; f < n; ++f) {
+      const ChannelData& like = *likes[f];
+      like.validate();
+      grids[f]->validate();
+      cfgs[f].beamformer.validate();
+      if (like.is_iq || like.rf.empty()) throw std::invalid_argument("FrameSequence: frames must be RF");
+      ChannelData tmp;
+      tmp.probe = like.probe;
+      tmp.medium = like.medium;
+      tmp.acq = like.acq;
+      tmp.transmits = like.transmits;
+      tmp.is_iq = false;
+      dplans_[f] = make_demod_plan(tmp, cfgs[f].demod);
+      bplans_[f] = make_bf_plan(like, dplans_[f].n_out, dplans_[f].fs_out, true, dplans_[f].fdem,
+                                *grids[f], cfgs[f].beamformer, 0, like.n_events());
+      auto it = rf_of_.find(likes[f]);
+      if (it == rf_of_.end()) {
+        void* p = alloc(like.rf.size() * sizeof(float));
+        check(c, usx_buffer_upload(c, p, like.rf.data(), like.rf.size() * sizeof(float)), "buffer_upload");
+        it = rf_of_.emplace(likes[f], static_cast<const float*>(p)).first;
+      }
+      rf[f] = it->second;
+      shapes_.push_back({grids[f]->n_axis0, grids[f]->n_axis1});
+      fr[f] = static_cast<usx_c32*>(alloc(sizeof(usx_c32) * grids[f]->n_pixels()));
+      frames_.push_back(fr[f]);
+    }
+    std::vector<usx_demod_desc> d(n);
+    std::vector<usx_bf_desc> b(n);
+    for (std::size_t f = 0; f < n; ++f) {
+      DemodPlan& dp = dplans_[f];
+      BeamformPlan& bp = bplans_[f];
+      dp.desc.taps = dp.taps.data();
+      dp.desc.rotator = reinterpret_cast<const usx_c32*>(dp.rotator.data());
+      bp.desc.element_x = bp.element_x.data();
+      bp.desc.tx = bp.tx.data();
+      bp.desc.axis0 = bp.axis0.data();
+      bp.desc.axis1 = bp.axis1.data();
+      bp.desc.axis0_sin = bp.s0.empty() ? nullptr : bp.s0.data();
+      bp.desc.axis0_cos = bp.c0.empty() ? nullptr : bp.c0.data();
+      d[f] = dp.desc;
+      b[f] = bp.desc;
+    }
+    py::gil_scoped_release release;
+    check(c, usx_plan_create(c, static_cast<int>(n), d.data(), b.data(), rf.data(), fr.data(), &plan_),
+          "plan_create");
+  }
+  ~FrameSequence() {
+    usx_ctx* c = ctx_.get();
+    if (plan_) usx_plan_destroy(c, plan_);
+    for (void* p : owned_) usx_buffer_free(c, p);
+  }
+  FrameSequence(const FrameSequence&) = delete;
+  FrameSequence& operator=(const FrameSequence&) = delete;
+
+  void launch(bool graph) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_plan_launch(c, plan_, graph ? 1 : 0), "plan_launch");
+  }
+  void synchronize() {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_ctx_synchronize(c), "synchronize");
+  }
+  void record(int slot) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_ctx_event_record(c, slot), "event_record");
+  }
+  float elapsed(int a, int b) {
+    usx_ctx* c = ctx_.get();
+    float ms = 0;
+    check(c, usx_ctx_event_elapsed(c, a, b, &ms), "event_elapsed");
+    return ms;
+  }
+  // Zero every output frame (a fill kernel on the sequence's stream).
+  void clear() {
+    usx_ctx* c = ctx_.get();
+    for (std::size_t f = 0; f < frames_.size(); ++f)
+      check(c, usx_buffer_zero(c, frames_[f], sizeof(usx_c32) * shapes_[f].first * shapes_[f].second),
+            "buffer_zero");
+  }
+  py::list frames() {
+    usx_ctx* c = ctx_.get();
+    py::list out;
+    for (std::size_t f = 0; f < frames_.size(); ++f) {
+      py::array_t<std::complex<float>> a({shapes_[f].first, shapes_[f].second});
+      check(c, usx_buffer_download(c, a.mutable_data(), frames_[f],
+                                   sizeof(usx_c32) * shapes_[f].first * shapes_[f].second),
+            "buffer_download");
+      out.append(a);
+    }
+    return out;
+  }
+  // Launch shape of each entry as planned: (slots, mv occupancy).
+  py::list tiles() const {
+    py::list out;
+    for (const auto& bp : bplans_) out.append(py::make_tuple(bp.desc.tile_slots, bp.desc.tile_mv_occupancy));
+    return out;
+  }
+  std::size_t size() const { return dplans_.size(); }
+
+ private:
+  void* alloc(std::size_t bytes) {
+    usx_ctx* c = ctx_.get();
+    void* p = nullptr;
+    check(c, usx_buffer_alloc(c, bytes, &p), "buffer_alloc");
+    owned_.push_back(p);
+    return p;
+  }
+  std::vector<DemodPlan> dplans_;
+  std::vector<BeamformPlan> bplans_;
+  std::map<const ChannelData*, const float*> rf_of_;
+  std::vector<usx_c32*> frames_;
+  std::vector<std::pair<py::ssize_t, py::ssize_t>> shapes_;
+  std::vector<void*> owned_;
+  GpuContext ctx_;
+  usx_plan* plan_ = nullptr;
 };
 
 }  // namespace
@@ -642,7 +822,9 @@ PYBIND11_MODULE(_usx, m) {
       .def_readwrite("mv_diagonal_loading", &BeamformerConfig::mv_diagonal_loading)
       .def_readwrite("mv_range_average", &BeamformerConfig::mv_range_average)
       .def_readwrite("mv_precision", &BeamformerConfig::mv_precision)
-      .def_readwrite("mv_refine", &BeamformerConfig::mv_refine);
+      .def_readwrite("mv_refine", &BeamformerConfig::mv_refine)
+      .def_readwrite("tile_slots", &BeamformerConfig::tile_slots)
+      .def_readwrite("tile_mv_occupancy", &BeamformerConfig::tile_mv_occupancy);
 
   py::class_<Beamformer>(m, "Beamformer")
       .def(py::init<BeamformerConfig>(), py::arg("config") = BeamformerConfig{})
@@ -776,7 +958,25 @@ PYBIND11_MODULE(_usx, m) {
            py::arg("lr") = 1e-3f, py::arg("beta1") = 0.9f, py::arg("beta2") = 0.999f,
            py::arg("eps") = 1e-8f, py::arg("weight_decay") = 0.0f, py::arg("seed") = 0ull)
       .def("gradient", &DeviceRunner::gradient, py::arg("x"), py::arg("g"), py::arg("mask"),
-           py::arg("sw"));
+           py::arg("sw"))
+      .def("autotune", &DeviceRunner::autotune, py::arg("input_ptr"), py::arg("frame_ptr"),
+           py::arg("reps") = 3)
+      .def("autotune_host", &DeviceRunner::autotune_host, py::arg("channel_data"),
+           py::arg("reps") = 3);
+
+  py::class_<FrameSequence>(m, "FrameSequence")
+      .def(py::init<const std::vector<ChannelData*>&, const std::vector<ScanGrid*>&,
+                    const std::vector<PipelineConfig>&>(),
+           py::arg("channel_data"), py::arg("grids"), py::arg("configs"))
+      .def("clear", &FrameSequence::clear)
+      .def("frames", &FrameSequence::frames)
+      .def("tiles", &FrameSequence::tiles)
+      .def("launch", &FrameSequence::launch, py::arg("graph") = true,
+           py::call_guard<py::gil_scoped_release>())
+      .def("synchronize", &FrameSequence::synchronize, py::call_guard<py::gil_scoped_release>())
+      .def("record", &FrameSequence::record, py::arg("slot"))
+      .def("elapsed", &FrameSequence::elapsed, py::arg("a"), py::arg("b"))
+      .def("__len__", &FrameSequence::size);
   m.attr("APOD_PARAMS") = USX_APOD_PARAMS;
 
   m.def("build_info", []() { return std::string(usx_build_info()); });

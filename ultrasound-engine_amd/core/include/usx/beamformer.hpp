@@ -34,6 +34,10 @@ struct BeamformerConfig {
   // reference's arithmetic.
   MvPrecision mv_precision = MvPrecision::FP32;
   bool mv_refine = true;  // one step of iterative refinement, FP64 residual
+  // Launch-shape tunables (0 = default): transmit events per beamforming
+  // launch (1..16) and persistent MV workgroups per CU; usx.autotune picks them.
+  int tile_slots = 0;
+  int tile_mv_occupancy = 0;
 
   void validate() const;
 };

@@ -199,6 +199,8 @@ BeamformPlan make_bf_plan(const ChannelData& data, int samples, real fs, bool is
   b.mv_range_average = cfg.mv_range_average;
   b.mv_precision = static_cast<int>(cfg.mv_precision);
   b.mv_refine = cfg.mv_refine ? 1 : 0;
+  b.tile_slots = cfg.tile_slots;
+  b.tile_mv_occupancy = cfg.tile_mv_occupancy;
   return p;
 }
 

@@ -180,6 +180,8 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->is_iq && !(b->demodulation_frequency > 0))
     fail(c, USX_ERR_INVALID, "IQ ChannelData needs a positive demodulation_frequency");
   if (b->mv_precision < 0 || b->mv_precision > 2) fail(c, USX_ERR_INVALID, "unknown mv_precision");
+  if (b->tile_slots < 0 || b->tile_slots > 16 || b->tile_mv_occupancy < 0 || b->tile_mv_occupancy > 32)
+    fail(c, USX_ERR_INVALID, "beamform: tile_slots must be in 0..16 and tile_mv_occupancy in 0..32");
   if (b->combiner == 2) {
     const int pmax = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray);
     if (pmax > usxd::kMvMaxP)
@@ -233,7 +235,17 @@ void drain_profile(usx_ctx* c) {
   c->pending.clear();
 }
 
-BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
+// Device copies of a beamform descriptor's host tables.
+struct BfTables {
+  const float* ex = nullptr;
+  const usx_tx_desc* tx = nullptr;
+  const float* ax0 = nullptr;
+  const float* ax1 = nullptr;
+  const float* s0 = nullptr;
+  const float* c0 = nullptr;
+};
+
+BfArgs args_of(const usx_bf_desc* b, const BfTables& t) {
   BfArgs A{};
   A.n_ch = b->n_channels;
   A.n_s = b->n_samples;
@@ -252,27 +264,47 @@ BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
   A.rx_window = b->rx_window;
   A.tukey_alpha = b->tukey_alpha;
   A.transmit_weighting = b->transmit_weighting;
-  A.ex = static_cast<const float*>(upload(c, c->ex, b->element_x, sizeof(float) * b->n_channels));
-  A.tx = static_cast<const usx_tx_desc*>(
-      upload(c, c->tx, b->tx, sizeof(usx_tx_desc) * b->n_events));
+  A.ex = t.ex;
+  A.tx = t.tx;
   A.g.type = b->grid_type;
   A.g.n0 = b->n_axis0;
   A.g.n1 = b->n_axis1;
-  A.g.axis0 = static_cast<const float*>(upload(c, c->ax0, b->axis0, sizeof(float) * b->n_axis0));
-  A.g.axis1 = static_cast<const float*>(upload(c, c->ax1, b->axis1, sizeof(float) * b->n_axis1));
-  if (b->grid_type == 1) {
-    A.g.s0 = static_cast<const float*>(upload(c, c->s0, b->axis0_sin, sizeof(float) * b->n_axis0));
-    A.g.c0 = static_cast<const float*>(upload(c, c->c0, b->axis0_cos, sizeof(float) * b->n_axis0));
-  } else {
-    A.g.s0 = A.g.axis0;
-    A.g.c0 = A.g.axis0;
-  }
+  A.g.axis0 = t.ax0;
+  A.g.axis1 = t.ax1;
+  A.g.s0 = b->grid_type == 1 ? t.s0 : t.ax0;
+  A.g.c0 = b->grid_type == 1 ? t.c0 : t.ax0;
   A.g.apex_x = b->apex_x;
   A.g.apex_z = b->apex_z;
   return A;
 }
 
-constexpr int kSlots = 16;
+BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
+  BfTables t;
+  t.ex = static_cast<const float*>(upload(c, c->ex, b->element_x, sizeof(float) * b->n_channels));
+  t.tx = static_cast<const usx_tx_desc*>(upload(c, c->tx, b->tx, sizeof(usx_tx_desc) * b->n_events));
+  t.ax0 = static_cast<const float*>(upload(c, c->ax0, b->axis0, sizeof(float) * b->n_axis0));
+  t.ax1 = static_cast<const float*>(upload(c, c->ax1, b->axis1, sizeof(float) * b->n_axis1));
+  if (b->grid_type == 1) {
+    t.s0 = static_cast<const float*>(upload(c, c->s0, b->axis0_sin, sizeof(float) * b->n_axis0));
+    t.c0 = static_cast<const float*>(upload(c, c->c0, b->axis0_cos, sizeof(float) * b->n_axis0));
+  }
+  return args_of(b, t);
+}
+
+// Launch-shape tunables of one beamforming call (what the autotuner picks):
+// event slots per beamforming launch (1..16) and persistent MV workgroups per
+// CU (0: as many as LDS allows). From the descriptor's tile fields; 0 = default.
+struct Tile {
+  int slots = 16;
+  int mv_occupancy = 0;
+};
+
+Tile tile_of(const usx_bf_desc* b) {
+  Tile t;
+  if (b->tile_slots > 0) t.slots = std::min(16, b->tile_slots);
+  t.mv_occupancy = std::max(0, b->tile_mv_occupancy);
+  return t;
+}
 
 usxd::ApodNet apod_net(usx_ctx* c) {
   usxd::ApodNet N{};
@@ -285,12 +317,57 @@ usxd::ApodNet apod_net(usx_ctx* c) {
   return N;
 }
 
-// Enqueue the beamforming of device-resident data into a device frame.
-// d_weights / d_status (MV only, may be null): the effective channel weights
-// and per-pixel status of the minimum-variance solve.
-void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* d_frame,
-                  float2* d_weights = nullptr, int* d_status = nullptr) {
-  const BfArgs A = make_args(c, b);
+// Device scratch of one beamforming launch sequence.
+struct Scratch {
+  float2* Y = nullptr;
+  uint32_t* mask = nullptr;
+  float* acc = nullptr;
+  int* counts = nullptr;
+  int* lists = nullptr;
+};
+
+// Bytes of each scratch buffer the launch sequence of b needs (0: unused).
+struct ScratchSize {
+  size_t y = 0, mask = 0, acc = 0, counts = 0, lists = 0;
+};
+
+ScratchSize scratch_size(const usx_bf_desc* b, const Tile& T) {
+  ScratchSize z;
+  const size_t n_pix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+  const int n_ev = b->event_end - b->event_begin;
+  if (n_ev == 0) return z;
+  const int comb = b->combiner;
+  const bool select = b->compounding == 2, incoh = b->compounding == 1, cf = b->coherence_factor != 0;
+  const bool need_y = comb >= 2 || incoh || (!select && !(comb == 0 && !cf) && n_ev > T.slots);
+  if (need_y) {
+    z.y = n_pix * b->n_channels * sizeof(float2);
+    z.mask = n_pix * ((b->n_channels + 31) / 32) * sizeof(uint32_t);
+  }
+  if (incoh) z.acc = n_pix * sizeof(float);
+  if (comb == 2) {
+    z.counts = 16 * sizeof(int);
+    z.lists = static_cast<size_t>(usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray) / 16) * n_pix * sizeof(int);
+  }
+  return z;
+}
+
+Scratch context_scratch(usx_ctx* c, const ScratchSize& z) {
+  Scratch S;
+  if (z.y) S.Y = static_cast<float2*>(ensure(c, c->Y, z.y));
+  if (z.mask) S.mask = static_cast<uint32_t*>(ensure(c, c->mask, z.mask));
+  if (z.acc) S.acc = static_cast<float*>(ensure(c, c->acc, z.acc));
+  if (z.counts) S.counts = static_cast<int*>(ensure(c, c->mv_counts, z.counts));
+  if (z.lists) S.lists = static_cast<int*>(ensure(c, c->mv_lists, z.lists));
+  return S;
+}
+
+// Enqueue the beamforming of device-resident data into a device frame:
+// kernel launches only (no allocation, copy or memset), so the sequence can
+// be captured into a HIP graph. d_weights / d_status (MV only, may be null):
+// the effective channel weights and per-pixel status of the MV solve.
+void enqueue_beamform(usx_ctx* c, const usx_bf_desc* b, const BfArgs& A, const Scratch& S,
+                      const Tile& T, const void* d_data, float2* d_frame, float2* d_weights,
+                      int* d_status) {
   const bool iq = b->is_iq != 0;
   const int n_pix = b->n_axis0 * b->n_axis1;
   const int n_ev = b->event_end - b->event_begin;
@@ -298,6 +375,7 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
   const int comb = b->combiner;
   const bool select = b->compounding == 2;
   const bool incoh = b->compounding == 1;
+  const int slots = T.slots;
   hipStream_t st = c->stream;
 
   if (n_ev == 0) {  // nothing to beamform: the reference leaves the frame zero
@@ -305,9 +383,8 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
     return;
   }
   const size_t ny = static_cast<size_t>(n_pix) * b->n_channels;
-  const size_t nmask = static_cast<size_t>(n_pix) * ((b->n_channels + 31) / 32);
-  auto Ybuf = [&]() { return static_cast<float2*>(ensure(c, c->Y, ny * sizeof(float2))); };
-  auto Mbuf = [&]() { return static_cast<uint32_t*>(ensure(c, c->mask, nmask * sizeof(uint32_t))); };
+  float2* Y = S.Y;
+  uint32_t* mk = S.mask;
 
   usxd::MvArgs M{};
   M.n0 = b->n_axis0;
@@ -319,9 +396,10 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
   M.cf = cf;
   M.precision = b->mv_precision;
   M.refine = b->mv_refine;
+  M.occupancy = T.mv_occupancy;
   if (comb == 3 && !c->apod_loaded)
     fail(c, USX_ERR_INVALID, "learned apodization: no network loaded (usx_apod_set_params)");
-  auto mv = [&](float2* Y, uint32_t* mk, float* acc) {
+  auto mv = [&](float* acc) {
     if (comb == 3) {
       timed(c, "apod_infer", [&] {
         usxd::launch_apod_infer(apod_net(c), n_pix, b->n_channels, Y, mk, d_frame, acc, cf,
@@ -329,47 +407,39 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
       });
       return;
     }
-    const int nb = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray) / 16;
-    int* counts = static_cast<int*>(ensure(c, c->mv_counts, 16 * sizeof(int)));
-    int* lists = static_cast<int*>(ensure(c, c->mv_lists, static_cast<size_t>(nb) * n_pix * sizeof(int)));
     M.weights = d_weights;
     M.status = d_status;
     if (d_weights)
       timed(c, "fill_zero", [&] { usxd::launch_fill_zero(reinterpret_cast<float*>(d_weights), 2 * ny, st); });
-    timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, counts, lists, st); });
+    timed(c, "mv", [&] { usxd::launch_mv(M, Y, mk, d_frame, acc, S.counts, S.lists, st); });
   };
 
   if (incoh) {
-    float* acc = static_cast<float*>(ensure(c, c->acc, n_pix * sizeof(float)));
-    timed(c, "fill_zero", [&] { usxd::launch_fill_zero(acc, n_pix, st); });
-    float2* Y = Ybuf();
-    uint32_t* mk = Mbuf();
+    timed(c, "fill_zero", [&] { usxd::launch_fill_zero(S.acc, n_pix, st); });
     for (int e = b->event_begin; e < b->event_end; ++e) {
       timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e, 1, 0, 1, comb >= 2, iq, st); });
       if (comb >= 2) {
         M.incoherent = 1;
-        mv(Y, mk, acc);
+        mv(S.acc);
       } else {
-        timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, acc, 1, comb, cf, st); });
+        timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, S.acc, 1, comb, cf, st); });
       }
     }
-    timed(c, "incoherent_finish", [&] { usxd::launch_incoherent_finish(n_pix, acc, d_frame, st); });
+    timed(c, "incoherent_finish", [&] { usxd::launch_incoherent_finish(n_pix, S.acc, d_frame, st); });
     return;
   }
 
   if (comb >= 2) {
-    float2* Y = Ybuf();
-    uint32_t* mk = Mbuf();
     if (select) {
       timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, b->event_begin, 1, 1, 1, 1, iq, st); });
     } else {
-      for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
-        const int n = std::min(kSlots, b->event_end - e0);
+      for (int e0 = b->event_begin; e0 < b->event_end; e0 += slots) {
+        const int n = std::min(slots, b->event_end - e0);
         timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e0, n, 0, e0 == b->event_begin, 1, iq, st); });
       }
     }
     M.incoherent = 0;
-    mv(Y, mk, nullptr);
+    mv(nullptr);
     return;
   }
 
@@ -378,33 +448,45 @@ void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* 
     return;
   }
   if (comb == 0 && !cf) {
-    for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
-      const int n = std::min(kSlots, b->event_end - e0);
+    for (int e0 = b->event_begin; e0 < b->event_end; e0 += slots) {
+      const int n = std::min(slots, b->event_end - e0);
       timed(c, "bf_fused", [&] { usxd::launch_fused(A, d_data, d_frame, e0, n, 0, e0 != b->event_begin, 0, false, iq, st); });
     }
     return;
   }
-  if (n_ev <= kSlots) {
+  if (n_ev <= slots) {
     timed(c, "bf_fused", [&] { usxd::launch_fused(A, d_data, d_frame, b->event_begin, n_ev, 0, 0, comb, cf, iq, st); });
     return;
   }
-  float2* Y = Ybuf();
-  uint32_t* mk = Mbuf();
-  for (int e0 = b->event_begin; e0 < b->event_end; e0 += kSlots) {
-    const int n = std::min(kSlots, b->event_end - e0);
+  for (int e0 = b->event_begin; e0 < b->event_end; e0 += slots) {
+    const int n = std::min(slots, b->event_end - e0);
     timed(c, "bf_channels", [&] { usxd::launch_channels(A, d_data, Y, mk, e0, n, 0, e0 == b->event_begin, 0, iq, st); });
   }
   timed(c, "bf_combine", [&] { usxd::launch_combine(n_pix, b->n_channels, Y, mk, d_frame, nullptr, 0, comb, cf, st); });
+}
+
+// Upload, size the context's scratch, enqueue.
+void run_beamform(usx_ctx* c, const usx_bf_desc* b, const void* d_data, float2* d_frame,
+                  float2* d_weights = nullptr, int* d_status = nullptr) {
+  const BfArgs A = make_args(c, b);
+  const Tile T = tile_of(b);
+  const Scratch S = context_scratch(c, scratch_size(b, T));
+  enqueue_beamform(c, b, A, S, T, d_data, d_frame, d_weights, d_status);
+}
+
+void enqueue_demod(usx_ctx* c, const usx_demod_desc* d, const float* taps, const float2* rot,
+                   const float* d_rf, float2* d_iq) {
+  timed(c, "demod", [&] {
+    usxd::launch_demod(d->n_traces, d->n_in, d->n_out, d->decimation, d->n_taps, taps, rot, d_rf,
+                       d_iq, c->stream);
+  });
 }
 
 void run_demod(usx_ctx* c, const usx_demod_desc* d, const float* d_rf, float2* d_iq) {
   const float* taps = static_cast<const float*>(upload(c, c->taps, d->taps, sizeof(float) * d->n_taps));
   const float2* rot =
       static_cast<const float2*>(upload(c, c->rot, d->rotator, sizeof(usx_c32) * d->n_in));
-  timed(c, "demod", [&] {
-    usxd::launch_demod(d->n_traces, d->n_in, d->n_out, d->decimation, d->n_taps, taps, rot, d_rf,
-                       d_iq, c->stream);
-  });
+  enqueue_demod(c, d, taps, rot, d_rf, d_iq);
 }
 
 void check_launch(usx_ctx* c, const char* what) { hip_check(c, hipGetLastError(), what); }
@@ -991,6 +1073,281 @@ usx_status usx_simulate(usx_ctx* c, const usx_sim_desc* d, int n_scat, const flo
                 "hipMemcpyAsync(rf)");
       hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     }
+  });
+}
+
+}  // extern "C"
+
+// ---- frame-sequence plans and the autotuner --------------------------------
+
+struct usx_plan {
+  int n = 0;
+  std::vector<usx_demod_desc> d;
+  std::vector<usx_bf_desc> b;
+  std::vector<DevBuf> tables;  // per frame: taps, rotator, element x, transmits, axes
+  std::vector<const float*> taps;
+  std::vector<const float2*> rot;
+  std::vector<BfArgs> args;
+  std::vector<const float*> rf;
+  std::vector<float2*> frames;
+  DevBuf iq, Y, mask, acc, counts, lists;
+  Scratch scratch;
+  float2* d_iq = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+void plan_release(usx_plan* p) {
+  if (!p) return;
+  if (!g_exiting) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (DevBuf& t : p->tables)
+      if (t.p) (void)hipFree(t.p);
+    for (DevBuf* t : {&p->iq, &p->Y, &p->mask, &p->acc, &p->counts, &p->lists})
+      if (t->p) (void)hipFree(t->p);
+  }
+  delete p;
+}
+
+// The whole launch sequence of a plan on the context stream (kernels only).
+void plan_enqueue(usx_ctx* c, usx_plan* p) {
+  for (int f = 0; f < p->n; ++f) {
+    enqueue_demod(c, &p->d[f], p->taps[f], p->rot[f], p->rf[f], p->d_iq);
+    enqueue_beamform(c, &p->b[f], p->args[f], p->scratch, tile_of(&p->b[f]), p->d_iq,
+                     p->frames[f], nullptr, nullptr);
+  }
+}
+
+// Copy the host tables of frame f into one device blob.
+void plan_tables(usx_ctx* c, usx_plan* p, int f) {
+  const usx_demod_desc& d = p->d[f];
+  const usx_bf_desc& b = p->b[f];
+  const size_t sz[8] = {sizeof(float) * d.n_taps, sizeof(usx_c32) * d.n_in,
+                        sizeof(float) * b.n_channels, sizeof(usx_tx_desc) * b.n_events,
+                        sizeof(float) * b.n_axis0, sizeof(float) * b.n_axis1,
+                        b.grid_type == 1 ? sizeof(float) * b.n_axis0 : 0,
+                        b.grid_type == 1 ? sizeof(float) * b.n_axis0 : 0};
+  const void* src[8] = {d.taps, d.rotator, b.element_x, b.tx, b.axis0, b.axis1, b.axis0_sin,
+                        b.axis0_cos};
+  size_t off[8], total = 0;
+  for (int k = 0; k < 8; ++k) {
+    off[k] = total;
+    total += align256(sz[k]);
+  }
+  std::vector<unsigned char> host(total, 0);
+  for (int k = 0; k < 8; ++k)
+    if (sz[k]) std::memcpy(host.data() + off[k], src[k], sz[k]);
+  DevBuf& t = p->tables[f];
+  ensure(c, t, total);
+  hip_check(c, hipMemcpy(t.p, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy(plan tables)");
+  auto at = [&](int k) { return static_cast<const void*>(static_cast<unsigned char*>(t.p) + off[k]); };
+  p->taps[f] = static_cast<const float*>(at(0));
+  p->rot[f] = static_cast<const float2*>(at(1));
+  BfTables bt;
+  bt.ex = static_cast<const float*>(at(2));
+  bt.tx = static_cast<const usx_tx_desc*>(at(3));
+  bt.ax0 = static_cast<const float*>(at(4));
+  bt.ax1 = static_cast<const float*>(at(5));
+  bt.s0 = static_cast<const float*>(at(6));
+  bt.c0 = static_cast<const float*>(at(7));
+  p->args[f] = args_of(&b, bt);
+}
+
+}  // namespace
+
+extern "C" {
+
+usx_status usx_buffer_alloc(usx_ctx* c, size_t bytes, void** dev) {
+  return guarded(c, [&] {
+    if (!dev) fail(c, USX_ERR_INVALID, "usx_buffer_alloc: null output");
+    *dev = nullptr;
+    hip_check(c, hipMalloc(dev, bytes == 0 ? 16 : bytes), "hipMalloc");
+  });
+}
+
+usx_status usx_buffer_free(usx_ctx* c, void* dev) {
+  if (!dev || g_exiting) return USX_OK;
+  return guarded(c, [&] { hip_check(c, hipFree(dev), "hipFree"); });
+}
+
+usx_status usx_buffer_upload(usx_ctx* c, void* dev, const void* host, size_t bytes) {
+  return guarded(c, [&] {
+    if (!dev || (!host && bytes)) fail(c, USX_ERR_INVALID, "usx_buffer_upload: null pointer");
+    hip_check(c, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(upload)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_buffer_download(usx_ctx* c, void* host, const void* dev, size_t bytes) {
+  return guarded(c, [&] {
+    if (!dev || (!host && bytes)) fail(c, USX_ERR_INVALID, "usx_buffer_download: null pointer");
+    hip_check(c, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(download)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_buffer_zero(usx_ctx* c, void* dev, size_t bytes) {
+  return guarded(c, [&] {
+    if (!dev || bytes % 4) fail(c, USX_ERR_INVALID, "usx_buffer_zero: null pointer or size not a multiple of 4");
+    usxd::launch_fill_zero(static_cast<float*>(dev), bytes / 4, c->stream);
+    check_launch(c, "fill launch");
+  });
+}
+
+usx_status usx_plan_create(usx_ctx* c, int n, const usx_demod_desc* d, const usx_bf_desc* b,
+                           const float* const* rf, usx_c32* const* frames, usx_plan** out) {
+  return guarded(c, [&] {
+    if (!out) fail(c, USX_ERR_INVALID, "usx_plan_create: null output");
+    *out = nullptr;
+    if (n <= 0 || !d || !b || !rf || !frames) fail(c, USX_ERR_INVALID, "usx_plan_create: empty frame list");
+    for (int f = 0; f < n; ++f) {
+      check_demod(c, &d[f]);
+      check_bf(c, &b[f]);
+      if (!b[f].is_iq || b[f].n_events * b[f].n_channels != d[f].n_traces || b[f].n_samples != d[f].n_out)
+        fail(c, USX_ERR_INVALID, "usx_plan_create: frame %d: demodulate and beamform shapes disagree", f);
+      if (!rf[f] || !frames[f]) fail(c, USX_ERR_INVALID, "usx_plan_create: frame %d: null device pointer", f);
+      if (b[f].combiner == 3 && !c->apod_loaded)
+        fail(c, USX_ERR_INVALID, "learned apodization: no network loaded (usx_apod_set_params)");
+    }
+    usx_plan* p = new usx_plan();
+    try {
+      p->n = n;
+      p->d.assign(d, d + n);
+      p->b.assign(b, b + n);
+      p->tables.resize(n);
+      p->taps.resize(n);
+      p->rot.resize(n);
+      p->args.resize(n);
+      p->rf.assign(rf, rf + n);
+      p->frames.resize(n);
+      for (int f = 0; f < n; ++f) p->frames[f] = reinterpret_cast<float2*>(frames[f]);
+      ScratchSize z;
+      size_t niq = 0;
+      for (int f = 0; f < n; ++f) {
+        plan_tables(c, p, f);
+        const ScratchSize s = scratch_size(&b[f], tile_of(&b[f]));
+        z.y = std::max(z.y, s.y);
+        z.mask = std::max(z.mask, s.mask);
+        z.acc = std::max(z.acc, s.acc);
+        z.counts = std::max(z.counts, s.counts);
+        z.lists = std::max(z.lists, s.lists);
+        niq = std::max(niq, static_cast<size_t>(d[f].n_traces) * d[f].n_out);
+      }
+      p->d_iq = static_cast<float2*>(ensure(c, p->iq, niq * sizeof(float2)));
+      if (z.y) p->scratch.Y = static_cast<float2*>(ensure(c, p->Y, z.y));
+      if (z.mask) p->scratch.mask = static_cast<uint32_t*>(ensure(c, p->mask, z.mask));
+      if (z.acc) p->scratch.acc = static_cast<float*>(ensure(c, p->acc, z.acc));
+      if (z.counts) p->scratch.counts = static_cast<int*>(ensure(c, p->counts, z.counts));
+      if (z.lists) p->scratch.lists = static_cast<int*>(ensure(c, p->lists, z.lists));
+      // one eager pass (first-use initialisation of the kernels happens outside
+      // the capture), then the capture of the whole sequence
+      const bool prof = c->profiling;
+      c->profiling = false;
+      plan_enqueue(c, p);
+      check_launch(c, "plan launch");
+      hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+      hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        plan_enqueue(c, p);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c->profiling = prof;
+        throw;
+      }
+      hip_check(c, hipStreamEndCapture(c->stream, &p->graph), "hipStreamEndCapture");
+      c->profiling = prof;
+      hip_check(c, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    } catch (...) {
+      plan_release(p);
+      throw;
+    }
+    *out = p;
+  });
+}
+
+usx_status usx_plan_launch(usx_ctx* c, usx_plan* p, int use_graph) {
+  return guarded(c, [&] {
+    if (!p) fail(c, USX_ERR_INVALID, "usx_plan_launch: null plan");
+    if (use_graph) {
+      hip_check(c, hipGraphLaunch(p->exec, c->stream), "hipGraphLaunch");
+    } else {
+      plan_enqueue(c, p);
+      check_launch(c, "plan launch");
+    }
+  });
+}
+
+usx_status usx_plan_destroy(usx_ctx* c, usx_plan* p) {
+  if (!p) return USX_OK;
+  if (c && !g_exiting) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+  }
+  plan_release(p);
+  return USX_OK;
+}
+
+usx_status usx_autotune(usx_ctx* c, const usx_demod_desc* d, usx_bf_desc* b, const float* rf,
+                        usx_c32* frame, int reps, float* best_ms) {
+  return guarded(c, [&] {
+    check_demod(c, d);
+    check_bf(c, b);
+    if (!rf || !frame) fail(c, USX_ERR_INVALID, "usx_autotune: null device pointer");
+    if (!b->is_iq || b->n_events * b->n_channels != d->n_traces || b->n_samples != d->n_out)
+      fail(c, USX_ERR_INVALID, "usx_autotune: demodulate and beamform shapes disagree");
+    reps = std::max(1, reps);
+    const int n_ev = b->event_end - b->event_begin;
+    std::vector<std::pair<int, int>> cand;  // (slots, mv occupancy)
+    const int slot_opts[3] = {4, 8, 16};
+    const int occ_opts[4] = {0, 4, 8, 16};
+    for (int s : slot_opts) {
+      if (s > 4 && s / 2 >= n_ev) continue;  // no more launches to merge
+      if (b->combiner == 2) {
+        for (int o : occ_opts) cand.push_back({s, o});
+      } else {
+        cand.push_back({s, 0});
+      }
+    }
+    const size_t niq = static_cast<size_t>(d->n_traces) * d->n_out;
+    float2* d_iq = static_cast<float2*>(ensure(c, c->iq, niq * sizeof(float2)));
+    const bool prof = c->profiling;
+    c->profiling = false;
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    float best = 1e30f;
+    std::pair<int, int> pick = {16, 0};
+    for (const auto& cd : cand) {
+      usx_bf_desc t = *b;
+      t.tile_slots = cd.first;
+      t.tile_mv_occupancy = cd.second;
+      run_demod(c, d, rf, d_iq);
+      run_beamform(c, &t, d_iq, reinterpret_cast<float2*>(frame));  // warm-up
+      hip_check(c, hipEventRecord(e0, c->stream), "hipEventRecord");
+      for (int r = 0; r < reps; ++r) {
+        run_demod(c, d, rf, d_iq);
+        run_beamform(c, &t, d_iq, reinterpret_cast<float2*>(frame));
+      }
+      hip_check(c, hipEventRecord(e1, c->stream), "hipEventRecord");
+      check_launch(c, "autotune launch");
+      hip_check(c, hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.0f;
+      hip_check(c, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      ms /= static_cast<float>(reps);
+      if (ms < best) {
+        best = ms;
+        pick = cd;
+      }
+    }
+    c->profiling = prof;
+    b->tile_slots = pick.first;
+    b->tile_mv_occupancy = pick.second;
+    if (best_ms) *best_ms = best;
   });
 }
 
