@@ -138,8 +138,10 @@ __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, boo
       h = A.n_ch - 1;
     }
   }
-  *lo = wave_min(l);
-  *hi = wave_max(h);
+  // wavefront-uniform (the reductions leave every lane the same value): keep
+  // them in scalar registers so the channel loop and its addresses are scalar
+  *lo = __builtin_amdgcn_readfirstlane(wave_min(l));
+  *hi = __builtin_amdgcn_readfirstlane(wave_max(h));
 }
 
 }  // namespace
@@ -171,17 +173,22 @@ __device__ __forceinline__ int tile_slots(const BfArgs& A, const Pt& p, bool val
 }
 
 // Sum over the event slots of one channel, then the receive rotation and
-// apodization: this lane's Y_ch (see the header).
+// apodization: this lane's Y_ch (see the header). `ch` is wavefront-uniform,
+// and so is the event of slot s unless `select` (then the slot's own event):
+// each slot's trace then starts at a scalar address and only the sample
+// index is per lane.
 template <int NE, bool IQ>
 __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
-                                                const TxSlot (&sl)[NE], int nsl, int ch, float a) {
+                                                const TxSlot (&sl)[NE], int nsl, int ch, float a,
+                                                int ev_first, int select) {
   const float trx = rx_time(A.ex[ch], p, A.c);
   float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     if (s < nsl) {
       const float idx = (sl[s].tau + trx - A.t0) * A.fs;
-      const size_t base = (static_cast<size_t>(sl[s].ev) * A.n_ch + ch) * A.n_s;
+      const int ev = select ? sl[s].ev : ev_first + s;
+      const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
       const float2 v = sample<IQ>(data, base, A.n_s, idx);
       acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
       acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
@@ -229,7 +236,7 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
     const float a = valid ? aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
     if (a <= 0.0f) continue;
     ++na;
-    const float2 y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a);
+    const float2 y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a, ev_first, select);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* 
           if (rect) a = a > 0.0f ? 1.0f : 0.0f;
           if (a > 0.0f) {
             atomicOr(&bits[lane][ch >> 5], 1u << (ch & 31));
-            y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a);
+            y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a, ev_first, select);
           }
         }
         tile[lane][cc] = y;
