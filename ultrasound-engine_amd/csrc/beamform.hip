@@ -104,20 +104,28 @@ __device__ __forceinline__ int select_event(const BfArgs& A, const Pt& p, float*
 }
 
 // Linear interpolation of a trace at a fractional index (beamformer.cpp:27-39);
-// zero when floor(idx) < 0 or floor(idx) + 1 >= n (and for NaN).
+// zero when floor(idx) < 0 or floor(idx) + 1 >= n (and for NaN). The load is
+// unconditional, at an index clamped into the trace, and the result is
+// selected afterwards: no branch around the load, so the loads of all event
+// slots can be in flight together (the value is the branchy version's, bit
+// for bit).
 template <bool IQ>
 __device__ __forceinline__ float2 sample(const void* data, size_t base, int n, float idx) {
-  if (!(idx >= 0.0f && idx < static_cast<float>(n - 1))) return make_float2(0.0f, 0.0f);
-  const int k = static_cast<int>(floorf(idx));
+  const bool ok = idx >= 0.0f && idx < static_cast<float>(n - 1);
+  const int k = ok ? static_cast<int>(floorf(idx)) : 0;
+  const int k1 = min(k + 1, n - 1);
   const float f = idx - static_cast<float>(k);
   const float g = 1.0f - f;
+  float2 r;
   if (IQ) {
     const float2* t = static_cast<const float2*>(data) + base;
-    const float2 a = t[k], b = t[k + 1];
-    return make_float2(fmaf(b.x, f, a.x * g), fmaf(b.y, f, a.y * g));
+    const float2 a = t[k], b = t[k1];
+    r = make_float2(fmaf(b.x, f, a.x * g), fmaf(b.y, f, a.y * g));
+  } else {
+    const float* t = static_cast<const float*>(data) + base;
+    r = make_float2(fmaf(t[k1], f, t[k] * g), 0.0f);
   }
-  const float* t = static_cast<const float*>(data) + base;
-  return make_float2(fmaf(t[k + 1], f, t[k] * g), 0.0f);
+  return ok ? r : make_float2(0.0f, 0.0f);
 }
 
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
@@ -152,7 +160,9 @@ template <int NE>
 __device__ __forceinline__ int tile_slots(const BfArgs& A, const Pt& p, bool valid, int ev_first,
                                           int n_slots, int select, TxSlot (&sl)[NE],
                                           TxSlot (*shared)[64]) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the wavefront index, uniform: kept scalar so channel indices and trace
+  // addresses derived from it are scalar too
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int nsl = n_slots;
   if (select) {
     nsl = 1;
@@ -174,24 +184,40 @@ __device__ __forceinline__ int tile_slots(const BfArgs& A, const Pt& p, bool val
 
 // Sum over the event slots of one channel, then the receive rotation and
 // apodization: this lane's Y_ch (see the header). `ch` is wavefront-uniform,
-// and so is the event of slot s unless `select` (then the slot's own event):
-// each slot's trace then starts at a scalar address and only the sample
-// index is per lane.
-template <int NE, bool IQ>
+// and so is the event of slot s unless SEL (Select compounding: the slot's
+// own event): each slot's trace then starts at a scalar address and only the
+// sample index is per lane.
+template <int NE, bool IQ, bool SEL>
 __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
                                                 const TxSlot (&sl)[NE], int nsl, int ch, float a,
-                                                int ev_first, int select) {
+                                                int ev_first) {
   const float trx = rx_time(A.ex[ch], p, A.c);
   float2 acc = make_float2(0.0f, 0.0f);
+  // slots in groups of four: one uniform branch per group, and inside it all
+  // four loads issued together (a slot past nsl reads slot 0's trace and is
+  // not added)
+  constexpr int G = NE < 4 ? NE : 4;
 #pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    if (s < nsl) {
-      const float idx = (sl[s].tau + trx - A.t0) * A.fs;
-      const int ev = select ? sl[s].ev : ev_first + s;
-      const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
-      const float2 v = sample<IQ>(data, base, A.n_s, idx);
-      acc.x = fmaf(v.x, sl[s].tr, fmaf(-v.y, sl[s].ti, acc.x));
-      acc.y = fmaf(v.x, sl[s].ti, fmaf(v.y, sl[s].tr, acc.y));
+  for (int s0 = 0; s0 < NE; s0 += G) {
+    if (s0 < nsl) {
+      float2 v[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const int s = s0 + q;
+        const bool on = s < nsl;
+        const float idx = (sl[on ? s : 0].tau + trx - A.t0) * A.fs;
+        const int ev = SEL ? sl[on ? s : 0].ev : ev_first + (on ? s : 0);
+        const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
+        v[q] = sample<IQ>(data, base, A.n_s, idx);
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const int s = s0 + q;
+        if (s < nsl) {
+          acc.x = fmaf(v[q].x, sl[s].tr, fmaf(-v[q].y, sl[s].ti, acc.x));
+          acc.y = fmaf(v[q].x, sl[s].ti, fmaf(v[q].y, sl[s].tr, acc.y));
+        }
+      }
     }
   }
   float rr = a, ri = 0.0f;
@@ -219,7 +245,9 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
   __shared__ float part[4][NPART][64];
   int i0, j;
   tile_of(A, &i0, &j);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the wavefront index, uniform: kept scalar so channel indices and trace
+  // addresses derived from it are scalar too
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool valid = i0 < A.g.n0 && j < A.g.n1;
   const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
   TxSlot sl[NE];
@@ -236,7 +264,8 @@ __global__ __launch_bounds__(256) void bf_fused_kernel(BfArgs A, const void* dat
     const float a = valid ? aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
     if (a <= 0.0f) continue;
     ++na;
-    const float2 y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a, ev_first, select);
+    const float2 y = select ? channel_value<NE, IQ, true>(A, data, p, sl, nsl, ch, a, ev_first)
+                            : channel_value<NE, IQ, false>(A, data, p, sl, nsl, ch, a, ev_first);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -318,7 +347,7 @@ __global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* 
   __shared__ uint32_t bits[64][kMaskWordsMax];
   int i0, j;
   tile_of(A, &i0, &j);
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool valid = i0 < A.g.n0 && j < A.g.n1;
   const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
   const int nw = (A.n_ch + 31) / 32;
@@ -340,7 +369,8 @@ __global__ __launch_bounds__(256) void bf_channels_kernel(BfArgs A, const void* 
           if (rect) a = a > 0.0f ? 1.0f : 0.0f;
           if (a > 0.0f) {
             atomicOr(&bits[lane][ch >> 5], 1u << (ch & 31));
-            y = channel_value<NE, IQ>(A, data, p, sl, nsl, ch, a, ev_first, select);
+            y = select ? channel_value<NE, IQ, true>(A, data, p, sl, nsl, ch, a, ev_first)
+                       : channel_value<NE, IQ, false>(A, data, p, sl, nsl, ch, a, ev_first);
           }
         }
         tile[lane][cc] = y;
