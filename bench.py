@@ -198,17 +198,57 @@ def run_stream(n_frames, cache_path=None):
     return res
 
 
+def pmc_workload(n_frames):
+    """The workload of the HBM-traffic counters: n MV frames of the benchmark
+    (device-resident RF) and, for calibration, one device copy of exactly
+    256 MiB. Under rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), traffic per
+    MV frame = counter sum over the MV-stage dispatches of a frame, times
+    (copy bytes / the copy's counter)."""
+    import torch
+    import usx
+    probe, med, acq, txs, ph, raw, grid = make_workload(usx)
+    pc = pipeline_config(usx, "MV")
+    runner = usx.DeviceRunner(grid, pc, raw)
+    rf = torch.from_numpy(np.ascontiguousarray(np.asarray(raw.rf))).to("cuda:0")
+    frame = torch.empty((grid.n_axis0, grid.n_axis1), dtype=torch.complex64, device="cuda:0")
+    for _ in range(n_frames):
+        runner.run(rf.data_ptr(), frame.data_ptr())
+    runner.synchronize()
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"pmc workload: {n_frames} MV frames, one {src.numel() * 4} B device copy")
+
+
+def measured_traffic():
+    """HBM bytes per MV frame from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "mv_stage_256x640_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_frame:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description="MI355X imaging-path benchmark")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--pmc-workload", type=int, default=0, metavar="N",
+                    help="run only N MV frames and a 256 MiB calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--skip-stream", action="store_true", help="omit the uneven-stream leg")
     ap.add_argument("--stream-frames", type=int, default=64)
     ap.add_argument("--tile-cache", default=None,
                     help="autotuner cache file (default: the package's usx/data/tile_cache.json)")
     a = ap.parse_args()
+    if a.pmc_workload:
+        pmc_workload(a.pmc_workload)
+        return 0
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the imaging path is "
                                    "data-parallel over frames and is measured per GPU",
@@ -265,7 +305,7 @@ def main():
     mv_kernel_ms = kern["mv"][1] / kern["mv"][0]
     achieved = flops / (mv_kernel_ms * 1e-3)
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA, "traffic": measured_traffic(),
             "kernel": "mv", "flops_per_launch": flops, "kernel_ms": mv_kernel_ms}
     # DAS is priced on compulsory HBM bytes (RF in + frame out) for reference.
     das_bytes = raw.rf.nbytes + grid.n_pixels * 8
