@@ -181,6 +181,7 @@ typedef struct {
   double demodulate_ms;
   double beamform_ms;
   double total_ms;
+  double postprocess_ms;   /* usx_process_display only */
 } usx_stage_times;
 
 /* RF in, frame out: demodulate (d) then beamform (b, whose n_samples,
@@ -189,6 +190,31 @@ typedef struct {
 usx_status usx_process_frame(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
                              const float* rf, usx_c32* frame, int memory,
                              usx_stage_times* times);
+
+/* ---- display tail (pipeline.cpp:48-79, postprocess.cpp:12-117) --------- */
+
+typedef struct {
+  int apply_tgc;                 /* PipelineConfig.apply_tgc */
+  float attenuation_db_cm_mhz;   /* Medium.attenuation */
+  float center_frequency_hz;     /* TGC frequency: the frame's demodulation frequency, or 5 MHz */
+  float dynamic_range_db, gain_db;
+  float reference_smoothing;     /* PipelineConfig.reference_smoothing */
+} usx_display_desc;
+
+/* Frame -> display image in [0, 1]: envelope (|IQ|; for an RF frame, rf = 1,
+ * the analytic-signal magnitude along axis1), TGC, log compression against
+ * the smoothed reference level, which *reference carries between frames
+ * (in/out, host; 0 = none yet) and is updated as ImagingPipeline::to_display
+ * does (replaces the body of to_display, pipeline.cpp:48-70). b supplies the
+ * grid as for usx_beamform. frame, image: n_axis0 * n_axis1. */
+usx_status usx_display(usx_ctx* ctx, const usx_bf_desc* b, const usx_display_desc* p,
+                       const usx_c32* frame, int rf, float* image, float* reference, int memory);
+/* RF in, display image out (ImagingPipeline::process_display,
+ * pipeline.cpp:72-79): usx_process_frame then usx_display, with the frame kept
+ * in device memory. times may be NULL. */
+usx_status usx_process_display(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
+                               const usx_display_desc* p, const float* rf, float* image,
+                               float* reference, int memory, usx_stage_times* times);
 
 /* ---- minimum-variance channel weights (no reference counterpart) ------- */
 

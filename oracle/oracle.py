@@ -35,6 +35,11 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 
 
+class OcDisplay(C.Structure):
+    _fields_ = [("apply_tgc", _i), ("dynamic_range_db", _f), ("gain_db", _f),
+                ("reference_smoothing", _f)]
+
+
 class OcSetup(C.Structure):
     _fields_ = [("n_elements", _i), ("pitch", _f), ("element_width", _f),
                 ("elevation_height", _f), ("center_frequency", _f), ("bandwidth", _f),
@@ -367,6 +372,19 @@ class Oracle:
                                              C.byref(b), event_begin, event_end,
                                              frame.ctypes.data_as(_fp)))
         return (frame, extra) if extra else frame
+
+    def display(self, setup: Setup, grid: GridSpec, frames, fdem: float, apply_tgc: bool = True,
+                dynamic_range_db: float = 60.0, gain_db: float = 0.0,
+                reference_smoothing: float = 0.85):
+        """Frames (k, n0, n1) complex64 through ImagingPipeline::to_display in
+        order; returns the display images (k, n0, n1) float32."""
+        fr = np.ascontiguousarray(np.asarray(frames, np.complex64).reshape(-1, grid.n0, grid.n1))
+        out = np.zeros(fr.shape, np.float32)
+        cfg = OcDisplay(int(apply_tgc), dynamic_range_db, gain_db, reference_smoothing)
+        s = setup.c()
+        self._check(self.lib.oc_display(C.byref(s), C.byref(grid.c()), C.byref(cfg), fr.shape[0],
+                                        fr.ctypes.data_as(_fp), _f(fdem), out.ctypes.data_as(_fp)))
+        return out
 
     def process_frame(self, setup: Setup, txs, rf, grid: GridSpec, demod: Demod, bf: Bf):
         arr, keep = self._txs(txs)

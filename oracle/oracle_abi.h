@@ -145,6 +145,19 @@ int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const 
                      const oc_grid* g, const oc_demod* d, const oc_bf* bf,
                      float* frame_out, double* stats_out);
 
+/* The display tail: frames through ImagingPipeline::to_display in order
+ * (pipeline.cpp:48-70; postprocess.cpp:12-117), so the smoothed reference
+ * level carries over from frame to frame. frames: n_frames * n0*n1 complex
+ * (interleaved); fdem > 0: IQ frames (envelope |.|, TGC at fdem), 0: RF
+ * frames (analytic signal along axis1, TGC at 5 MHz); attenuation from s.
+ * images_out: n_frames * n0*n1 floats. */
+typedef struct {
+  int apply_tgc;
+  float dynamic_range_db, gain_db, reference_smoothing;
+} oc_display_cfg;
+int oc_display(const oc_setup* s, const oc_grid* g, const oc_display_cfg* p, int n_frames,
+               const float* frames, float fdem, float* images_out);
+
 #ifdef __cplusplus
 }
 #endif

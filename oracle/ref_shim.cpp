@@ -292,4 +292,28 @@ int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const 
   });
 }
 
+int oc_display(const oc_setup* s, const oc_grid* g, const oc_display_cfg* p, int n_frames,
+               const float* frames, float fdem, float* images_out) {
+  return guarded([&] {
+    usx::PipelineConfig pc;
+    pc.apply_tgc = p->apply_tgc != 0;
+    pc.dynamic_range_db = p->dynamic_range_db;
+    pc.gain_db = p->gain_db;
+    pc.reference_smoothing = p->reference_smoothing;
+    const usx::ScanGrid grid = to_grid(g);
+    usx::ImagingPipeline pipe(grid, pc);
+    const usx::Medium med = to_medium(s);
+    const std::size_t n = static_cast<std::size_t>(grid.n_axis0) * grid.n_axis1;
+    for (int f = 0; f < n_frames; ++f) {
+      usx::Frame fr;
+      fr.grid = grid;
+      fr.demodulation_frequency = fdem;
+      fr.data.resize(n);
+      std::memcpy(static_cast<void*>(fr.data.data()), frames + 2 * n * f, n * sizeof(std::complex<float>));
+      const std::vector<usx::real> img = pipe.to_display(fr, med);
+      std::memcpy(images_out + n * f, img.data(), n * sizeof(float));
+    }
+  });
+}
+
 }  // extern "C"

@@ -664,6 +664,70 @@ void beamform(const oc_setup& s, const oc_transmit* txs, int n_tx, bool is_iq, c
   }
 }
 
+// ---- display tail (postprocess.cpp:12-117, pipeline.cpp:48-70) -------------
+
+// Iterative radix-2 FFT with the twiddle carried by complex multiplication,
+// in float, as the reference computes it.
+void fft(std::vector<cf32>& a, bool inverse) {
+  const std::size_t n = a.size();
+  if (n < 2) return;
+  for (std::size_t i = 1, j = 0; i < n; ++i) {
+    std::size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (std::size_t len = 2; len <= n; len <<= 1) {
+    const float ang = 2.0f * kPiF / static_cast<float>(len) * (inverse ? 1.0f : -1.0f);
+    const cf32 step(std::cos(ang), std::sin(ang));
+    for (std::size_t i = 0; i < n; i += len) {
+      cf32 w(1, 0);
+      for (std::size_t k = 0; k < len / 2; ++k) {
+        const cf32 u = a[i + k];
+        const cf32 v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+        w *= step;
+      }
+    }
+  }
+  if (inverse)
+    for (cf32& v : a) v /= static_cast<float>(n);
+}
+
+// envelope + TGC of one frame (postprocess.cpp:40-98)
+std::vector<float> display_envelope(const Grid& G, const cf32* frame, float fdem, int apply_tgc,
+                                    float atten) {
+  const std::size_t n = static_cast<std::size_t>(G.n0) * G.n1;
+  std::vector<float> env(n);
+  if (fdem > 0.0f) {
+    for (std::size_t k = 0; k < n; ++k) env[k] = std::abs(frame[k]);
+  } else {
+    std::size_t nf = 1;
+    while (nf < static_cast<std::size_t>(G.n1)) nf <<= 1;
+    std::vector<cf32> buf(nf);
+    for (int i = 0; i < G.n0; ++i) {
+      std::fill(buf.begin(), buf.end(), cf32(0, 0));
+      for (int j = 0; j < G.n1; ++j) buf[j] = cf32(frame[static_cast<std::size_t>(i) * G.n1 + j].real(), 0.0f);
+      fft(buf, false);
+      for (std::size_t k = 1; k < nf / 2; ++k) buf[k] *= 2.0f;
+      for (std::size_t k = nf / 2 + 1; k < nf; ++k) buf[k] = cf32(0, 0);
+      fft(buf, true);
+      for (int j = 0; j < G.n1; ++j) env[static_cast<std::size_t>(i) * G.n1 + j] = std::abs(buf[j]);
+    }
+  }
+  if (apply_tgc) {
+    const float f0 = fdem > 0.0f ? fdem : 5.0e6f;
+    const float f_mhz = f0 * 1e-6f;
+    for (std::size_t k = 0; k < n; ++k) {
+      const float depth_cm = G.pts[k].z * 100.0f;
+      const float gain_db = 2.0f * atten * f_mhz * std::max(depth_cm, 0.0f);
+      env[k] = env[k] * std::pow(10.0f, gain_db / 20.0f);
+    }
+  }
+  return env;
+}
+
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -780,6 +844,42 @@ int oc_beamform(const oc_setup* s, const oc_transmit* txs, int n_tx, int is_iq,
                 int event_begin, int event_end, float* frame_out) {
   return oc_beamform_ex(s, txs, n_tx, is_iq, data, fdem, g, bf, event_begin, event_end,
                         frame_out, 0, nullptr, nullptr, nullptr);
+}
+
+int oc_display(const oc_setup* s, const oc_grid* g, const oc_display_cfg* p, int n_frames,
+               const float* frames, float fdem, float* images_out) {
+  return guarded([&] {
+    const Grid G = build_grid(*g);
+    const std::size_t n = static_cast<std::size_t>(G.n0) * G.n1;
+    float reference = 0.0f;
+    for (int f = 0; f < n_frames; ++f) {
+      const cf32* fr = reinterpret_cast<const cf32*>(frames) + n * f;
+      const std::vector<float> env = display_envelope(G, fr, fdem, p->apply_tgc, s->attenuation);
+      float peak = 0.0f;
+      for (float v : env) peak = std::max(peak, v);
+      if (peak > 0.0f) {
+        const float a = std::clamp(p->reference_smoothing, 0.0f, 0.999f);
+        reference = (reference > 0.0f) ? (a * reference + (1.0f - a) * peak) : peak;
+      }
+      // log_compress(env, dr, gain, reference > 0 ? reference : -1)
+      float ref = reference > 0.0f ? reference : -1.0f;
+      if (ref <= 0.0f) {
+        ref = 0.0f;
+        for (float v : env) ref = std::max(ref, v);
+      }
+      float* out = images_out + n * f;
+      if (ref <= 0.0f) {
+        std::fill(out, out + n, 0.0f);
+        continue;
+      }
+      const float dr = std::max(p->dynamic_range_db, 1.0f);
+      for (std::size_t k = 0; k < n; ++k) {
+        const float v = env[k] / ref;
+        const float db = (v > 0.0f) ? 20.0f * std::log10(v) : -1e6f;
+        out[k] = std::clamp((db + p->gain_db + dr) / dr, 0.0f, 1.0f);
+      }
+    }
+  });
 }
 
 int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const float* rf,

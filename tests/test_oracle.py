@@ -312,3 +312,46 @@ def test_mv_against_lapack_and_weight_identity(port, O, golden):
     # out == sum_c g_c Y[c] with the effective per-channel weights
     wts = ex["weights"].astype(np.complex128).reshape(n0, n1, -1)
     assert rel_l2((wts * y).sum(-1), f64) < 1e-6
+
+
+def test_display_matches_reference_and_closed_forms(O):
+    """The display tail (pipeline.cpp:48-70 over postprocess.cpp:12-117):
+    the restatement equals the compiled reference bit for bit (IQ frames,
+    RF frames through the FFT analytic signal, a sector grid, the smoothed
+    reference over 3 frames), and obeys closed forms."""
+    port = O.load("port")
+    s = O.Setup()
+    rng = np.random.default_rng(1)
+    cases = [(O.GridSpec(0, -0.01, 0.01, 12, 0.005, 0.04, 100), 5e6),
+             (O.GridSpec(0, -0.01, 0.01, 9, 0.005, 0.04, 77), 0.0),
+             (O.GridSpec(1, -0.5, 0.5, 10, 0.01, 0.06, 64), 3e6)]
+    for gs, fdem in cases:
+        fr = (rng.normal(size=(3, gs.n0, gs.n1)) + 1j * rng.normal(size=(3, gs.n0, gs.n1))).astype(np.complex64)
+        fr *= np.array([1, 5, 2], np.float32)[:, None, None]
+        got = port.display(s, gs, fr, fdem)
+        if O.available("reference"):
+            ref = O.load("reference").display(s, gs, fr, fdem)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (gs.type, fdem)
+        # first frame: the reference level is its own peak, so the brightest
+        # pixel shows 1.0 and a pixel at -dr/2 dB below it 0.5
+        assert got[0].max() == 1.0
+    # closed forms on an IQ frame without TGC: env = |y|
+    gs = O.GridSpec(0, -0.01, 0.01, 4, 0.01, 0.02, 8)
+    fr = np.full((1, 4, 8), 1.0, np.complex64)
+    fr[0, 0, 0] = 10.0
+    fr[0, 1, 1] = 10.0 * 10 ** (-30 / 20)
+    fr[0, 2, 2] = 0.0
+    img = port.display(s, gs, fr, 5e6, apply_tgc=False, dynamic_range_db=60.0)[0]
+    assert img[0, 0] == 1.0
+    assert img[1, 1] == pytest.approx(0.5, abs=1e-6)
+    assert img[2, 2] == 0.0
+    assert img[3, 3] == pytest.approx((20 * np.log10(0.1) + 60) / 60, abs=1e-6)
+    # TGC: the gain at depth z is 10^(2 alpha f z_cm / 20) (no smoothing effect on the ratio)
+    img_t = port.display(s, gs, np.ones((1, 4, 8), np.complex64), 5e6, apply_tgc=True,
+                         dynamic_range_db=200.0)[0]
+    z = np.linspace(0.01, 0.02, 8)
+    gain_db = 2 * 0.5 * 5.0 * z * 100
+    want = (gain_db - gain_db.max() + 200.0) / 200.0
+    assert np.allclose(img_t[0], want, atol=2e-6)
+    # silence stays black
+    assert not port.display(s, gs, np.zeros((2, 4, 8), np.complex64), 5e6).any()

@@ -204,6 +204,43 @@ BeamformPlan make_bf_plan(const ChannelData& data, int samples, real fs, bool is
   return p;
 }
 
+BeamformPlan make_grid_plan(const ScanGrid& grid) {
+  BeamformPlan p;
+  p.axis0 = grid.axis0;
+  p.axis1 = grid.axis1;
+  if (grid.type == GridType::Sector) {
+    p.s0.resize(grid.n_axis0);
+    p.c0.resize(grid.n_axis0);
+    for (int i = 0; i < grid.n_axis0; ++i) {
+      p.s0[i] = std::sin(grid.axis0[i]);
+      p.c0[i] = std::cos(grid.axis0[i]);
+    }
+  }
+  usx_bf_desc& b = p.desc;
+  b.grid_type = static_cast<int>(grid.type);
+  b.n_axis0 = grid.n_axis0;
+  b.n_axis1 = grid.n_axis1;
+  b.axis0 = p.axis0.data();
+  b.axis1 = p.axis1.data();
+  b.axis0_sin = p.s0.empty() ? nullptr : p.s0.data();
+  b.axis0_cos = p.c0.empty() ? nullptr : p.c0.data();
+  b.apex_x = grid.apex.x;
+  b.apex_z = grid.apex.z;
+  return p;
+}
+
+// TGC frequency: the frame's demodulation frequency, else 5 MHz (pipeline.cpp:52-55).
+usx_display_desc make_display_desc(const PipelineConfig& cfg, const Medium& medium, real fdem) {
+  usx_display_desc d{};
+  d.apply_tgc = cfg.apply_tgc ? 1 : 0;
+  d.attenuation_db_cm_mhz = medium.attenuation;
+  d.center_frequency_hz = fdem > 0.0f ? fdem : 5.0e6f;
+  d.dynamic_range_db = cfg.dynamic_range_db;
+  d.gain_db = cfg.gain_db;
+  d.reference_smoothing = cfg.reference_smoothing;
+  return d;
+}
+
 ChannelData demodulate(const ChannelData& rf, DemodConfig config) {
   rf.validate();
   if (rf.is_iq) return rf;

@@ -8,6 +8,7 @@
 
 #include "usx/beamformer.hpp"
 #include "usx/demodulate.hpp"
+#include "usx/pipeline.hpp"
 #include "usx/types.hpp"
 #include "usx_gpu.h"
 
@@ -52,5 +53,8 @@ struct BeamformPlan {
 // (they differ from data.acq when the data is demodulated on the device).
 BeamformPlan make_bf_plan(const ChannelData& data, int samples, real fs, bool is_iq, real fdem,
                           const ScanGrid& grid, const BeamformerConfig& cfg, int ev0, int ev1);
+// Only the grid fields (the display tail needs no channel data).
+BeamformPlan make_grid_plan(const ScanGrid& grid);
+usx_display_desc make_display_desc(const PipelineConfig& cfg, const Medium& medium, real fdem);
 
 }  // namespace usx

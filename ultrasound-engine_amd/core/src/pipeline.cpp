@@ -90,28 +90,48 @@ Frame ImagingPipeline::process_frame(const ChannelData& data) {
 
 std::vector<real> ImagingPipeline::to_display(const Frame& frame, const Medium& medium) {
   const auto t = Clock::now();
-  std::vector<real> env = envelope(frame);
-  if (config_.apply_tgc) {
-    real f0 = frame.demodulation_frequency;
-    if (f0 <= 0.0f) f0 = 5.0e6f;
-    env = apply_tgc(env, frame.grid, medium.attenuation, f0);
-  }
-  real peak = 0.0f;
-  for (real v : env) peak = std::max(peak, v);
-  if (peak > 0.0f) {
-    const real a = std::clamp(config_.reference_smoothing, 0.0f, 0.999f);
-    reference_ = (reference_ > 0.0f) ? (a * reference_ + (1.0f - a) * peak) : peak;
-  }
-  std::vector<real> img = log_compress(env, config_.dynamic_range_db, config_.gain_db,
-                                       reference_ > 0.0f ? reference_ : -1.0f);
+  const BeamformPlan gp = make_grid_plan(frame.grid);
+  const usx_display_desc dd = make_display_desc(config_, medium, frame.demodulation_frequency);
+  std::vector<real> img(static_cast<std::size_t>(frame.grid.n_axis0) * frame.grid.n_axis1);
+  if (frame.data.size() != img.size()) throw std::invalid_argument("to_display: frame data does not match its grid");
+  usx_ctx* ctx = gpu_->get();
+  check(ctx, usx_display(ctx, &gp.desc, &dd, reinterpret_cast<const usx_c32*>(frame.data.data()),
+                         frame.demodulation_frequency > 0.0f ? 0 : 1, img.data(), &reference_, USX_HOST),
+        "display");
   stats_.postprocess_ms += ms_since(t);
   stats_.total_ms += stats_.postprocess_ms;
   return img;
 }
 
 std::vector<real> ImagingPipeline::process_display(const ChannelData& data) {
-  const Frame frame = process_frame(data);
-  return to_display(frame, data.medium);
+  if (data.is_iq) {
+    const Frame frame = process_frame(data);
+    return to_display(frame, data.medium);
+  }
+  // RF: demodulate, beamform and display on the device; the frame stays in HBM
+  const auto t_start = Clock::now();
+  stats_ = PipelineStats{};
+  data.validate();
+  config_.beamformer.validate();
+  if (data.rf.empty()) throw std::invalid_argument("demodulate: input has no RF samples");
+  usx_ctx* ctx = gpu_->get();
+  BeamformerConfig bcfg = config_.beamformer;
+  if (bcfg.combiner == Combiner::Learned && !has_apod_) bcfg.combiner = Combiner::MV;
+  const DemodPlan dp = make_demod_plan(data, config_.demod);
+  const BeamformPlan bp = make_bf_plan(data, dp.n_out, dp.fs_out, true, dp.fdem, grid_, bcfg, 0,
+                                       data.n_events());
+  const usx_display_desc dd = make_display_desc(config_, data.medium, dp.fdem);
+  std::vector<real> img(static_cast<std::size_t>(grid_.n_axis0) * grid_.n_axis1);
+  usx_stage_times t{};
+  check(ctx, usx_process_display(ctx, &dp.desc, &bp.desc, &dd, data.rf.data(), img.data(), &reference_,
+                                 USX_HOST, &t),
+        "process_display");
+  stats_.demodulate_ms = t.demodulate_ms;
+  stats_.beamform_ms = t.beamform_ms;
+  stats_.postprocess_ms = t.postprocess_ms;
+  ++frame_count_;
+  stats_.total_ms = ms_since(t_start);
+  return img;
 }
 
 }  // namespace usx

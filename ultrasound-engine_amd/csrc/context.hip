@@ -71,6 +71,8 @@ struct usx_ctx {
   // simulator
   Cached s_ex, s_delays, s_firing, s_fw, s_nf, s_pulse, s_noise;
   DevBuf s_scat, s_live, s_t0, s_n, s_pos, s_wave, s_partial, s_bmax, s_rf;
+  // display tail
+  DevBuf disp_env, disp_bmax, disp_img, disp_frame;
 };
 
 namespace {
@@ -588,7 +590,8 @@ usx_status usx_ctx_destroy(usx_ctx* c) {
                     &c->ds_mask, &c->ds_sw, &c->tr_h, &c->tr_z, &c->tr_dz, &c->tr_partial,
                     &c->tr_db, &c->tr_loss_part, &c->tr_losses, &c->tr_grads, &c->mv_w,
                     &c->mv_status, &c->ap_pixels, &c->s_scat, &c->s_live, &c->s_t0, &c->s_n,
-                    &c->s_pos, &c->s_wave, &c->s_partial, &c->s_bmax, &c->s_rf})
+                    &c->s_pos, &c->s_wave, &c->s_partial, &c->s_bmax, &c->s_rf, &c->disp_env,
+                    &c->disp_bmax, &c->disp_img, &c->disp_frame})
     if (b->p) (void)hipFree(b->p);
   for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0, &c->s_ex,
                     &c->s_delays, &c->s_firing, &c->s_fw, &c->s_nf, &c->s_pulse, &c->s_noise})
@@ -1348,6 +1351,150 @@ usx_status usx_autotune(usx_ctx* c, const usx_demod_desc* d, usx_bf_desc* b, con
     b->tile_slots = pick.first;
     b->tile_mv_occupancy = pick.second;
     if (best_ms) *best_ms = best;
+  });
+}
+
+}  // extern "C"
+
+// ---- display tail -----------------------------------------------------------
+
+namespace {
+
+usxd::DisplayArgs display_args(usx_ctx* c, const usx_bf_desc* b, const usx_display_desc* p) {
+  const BfArgs A = make_args(c, b);
+  usxd::DisplayArgs D{};
+  D.g = A.g;
+  D.apply_tgc = p->apply_tgc;
+  D.attenuation = p->attenuation_db_cm_mhz;
+  D.f_mhz = p->center_frequency_hz * 1e-6f;
+  D.dynamic_range_db = p->dynamic_range_db;
+  D.gain_db = p->gain_db;
+  return D;
+}
+
+// Envelope + TGC of a device frame, the peak to the host, the smoothed
+// reference updated (pipeline.cpp:58-66), then the log compression.
+void run_display(usx_ctx* c, const usx_bf_desc* b, const usx_display_desc* p, const float2* d_frame,
+                 bool rf, float* d_image, float* reference) {
+  if (rf && b->n_axis1 > 8192)
+    fail(c, USX_ERR_INVALID, "display: RF frames are limited to 8192 samples along axis1");
+  const usxd::DisplayArgs D = display_args(c, b, p);
+  const size_t n_pix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+  const int nb = usxd::display_blocks(D, rf);
+  float* env = static_cast<float*>(ensure(c, c->disp_env, n_pix * sizeof(float)));
+  float* bmax = static_cast<float*>(ensure(c, c->disp_bmax, (nb + 2) * sizeof(float)));
+  float* ref_dev = bmax + nb;
+  timed(c, "display_envelope", [&] { usxd::launch_display_envelope(D, d_frame, rf, env, bmax, c->stream); });
+  // the frame's peak from the block maxima, on the host, for the smoothing
+  std::vector<float> host(nb);
+  hip_check(c, hipMemcpyAsync(host.data(), bmax, nb * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpyAsync(peaks)");
+  hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  float peak = 0.0f;
+  for (float v : host) peak = std::max(peak, v);
+  float ref = *reference;
+  if (peak > 0.0f) {
+    const float a = std::clamp(p->reference_smoothing, 0.0f, 0.999f);
+    ref = (ref > 0.0f) ? (a * ref + (1.0f - a) * peak) : peak;
+  }
+  *reference = ref;
+  const float use = ref > 0.0f ? ref : -1.0f;
+  hip_check(c, hipMemcpyAsync(ref_dev, &use, sizeof(float), hipMemcpyHostToDevice, c->stream),
+            "hipMemcpyAsync(reference)");
+  timed(c, "display_compress", [&] { usxd::launch_display_compress(D, env, bmax, nb, ref_dev, d_image, c->stream); });
+}
+
+// The grid fields of a beamform descriptor (the display tail needs no more).
+void check_grid(usx_ctx* c, const usx_bf_desc* b) {
+  if (!b) fail(c, USX_ERR_INVALID, "display: null grid descriptor");
+  if (b->n_axis0 <= 0 || b->n_axis1 <= 0) fail(c, USX_ERR_INVALID, "ScanGrid dimensions must be > 0");
+  if (!b->axis0 || !b->axis1) fail(c, USX_ERR_INVALID, "display: grid axes are required");
+  if (b->grid_type != 0 && b->grid_type != 1) fail(c, USX_ERR_INVALID, "beamform: unknown grid type");
+  if (b->grid_type == 1 && (!b->axis0_sin || !b->axis0_cos))
+    fail(c, USX_ERR_INVALID, "beamform: sector grids need the sin/cos tables");
+  if (b->n_events < 0 || b->n_channels < 0) fail(c, USX_ERR_INVALID, "display: negative sizes");
+}
+
+void check_display(usx_ctx* c, const usx_display_desc* p, const float* reference) {
+  if (!p || !reference) fail(c, USX_ERR_INVALID, "display: null descriptor or reference");
+}
+
+}  // namespace
+
+extern "C" {
+
+usx_status usx_display(usx_ctx* c, const usx_bf_desc* b, const usx_display_desc* p,
+                       const usx_c32* frame, int rf, float* image, float* reference, int memory) {
+  return guarded(c, [&] {
+    check_grid(c, b);
+    check_display(c, p, reference);
+    if (!frame || !image) fail(c, USX_ERR_INVALID, "usx_display: null data pointer");
+    const size_t n_pix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+    if (memory == USX_DEVICE) {
+      run_display(c, b, p, reinterpret_cast<const float2*>(frame), rf != 0, image, reference);
+      check_launch(c, "display launch");
+      return;
+    }
+    float2* d_frame = static_cast<float2*>(ensure(c, c->out, n_pix * sizeof(float2)));
+    float* d_image = static_cast<float*>(ensure(c, c->disp_img, n_pix * sizeof(float)));
+    hip_check(c, hipMemcpyAsync(d_frame, frame, n_pix * sizeof(float2), hipMemcpyHostToDevice, c->stream),
+              "hipMemcpyAsync(frame)");
+    run_display(c, b, p, d_frame, rf != 0, d_image, reference);
+    check_launch(c, "display launch");
+    hip_check(c, hipMemcpyAsync(image, d_image, n_pix * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpyAsync(image)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+usx_status usx_process_display(usx_ctx* c, const usx_demod_desc* d, const usx_bf_desc* b,
+                               const usx_display_desc* p, const float* rf, float* image,
+                               float* reference, int memory, usx_stage_times* t) {
+  return guarded(c, [&] {
+    check_demod(c, d);
+    check_bf(c, b);
+    check_display(c, p, reference);
+    if (!rf || !image) fail(c, USX_ERR_INVALID, "usx_process_display: null data pointer");
+    if (!b->is_iq || b->n_events * b->n_channels != d->n_traces || b->n_samples != d->n_out)
+      fail(c, USX_ERR_INVALID, "usx_process_display: demodulate and beamform shapes disagree");
+    const size_t nin = static_cast<size_t>(d->n_traces) * d->n_in;
+    const size_t nout = static_cast<size_t>(d->n_traces) * d->n_out;
+    const size_t n_pix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
+    const auto w0 = std::chrono::steady_clock::now();
+    hip_check(c, hipEventRecord(c->ev[0], c->stream), "hipEventRecord");
+    const float* d_rf = rf;
+    float* d_image = image;
+    if (memory != USX_DEVICE) {
+      float* q = static_cast<float*>(ensure(c, c->in, nin * sizeof(float)));
+      hip_check(c, hipMemcpyAsync(q, rf, nin * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                "hipMemcpyAsync(rf)");
+      d_rf = q;
+      d_image = static_cast<float*>(ensure(c, c->disp_img, n_pix * sizeof(float)));
+    }
+    float2* d_iq = static_cast<float2*>(ensure(c, c->iq, nout * sizeof(float2)));
+    float2* d_frame = static_cast<float2*>(ensure(c, c->disp_frame, n_pix * sizeof(float2)));
+    run_demod(c, d, d_rf, d_iq);
+    hip_check(c, hipEventRecord(c->ev[1], c->stream), "hipEventRecord");
+    run_beamform(c, b, d_iq, d_frame);
+    check_launch(c, "beamform launch");
+    hip_check(c, hipEventRecord(c->ev[2], c->stream), "hipEventRecord");
+    run_display(c, b, p, d_frame, false, d_image, reference);
+    check_launch(c, "display launch");
+    hip_check(c, hipEventRecord(c->ev[3], c->stream), "hipEventRecord");
+    if (memory != USX_DEVICE)
+      hip_check(c, hipMemcpyAsync(image, d_image, n_pix * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                "hipMemcpyAsync(image)");
+    hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (t) {
+      float a = 0, bb = 0, cc = 0;
+      hip_check(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "hipEventElapsedTime");
+      hip_check(c, hipEventElapsedTime(&bb, c->ev[1], c->ev[2]), "hipEventElapsedTime");
+      hip_check(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]), "hipEventElapsedTime");
+      t->demodulate_ms = a;
+      t->beamform_ms = bb;
+      t->postprocess_ms = cc;
+      t->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
   });
 }
 

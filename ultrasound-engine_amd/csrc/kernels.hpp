@@ -161,4 +161,22 @@ int sim_chunks(int n_scat);
 void launch_sim_event(const SimArgs& S, int ev, const float* scat, int n_scat, const SimIncident& I,
                       float* partial, float* blockmax, const float* z, float* rf, hipStream_t st);
 
+// display.hip: envelope, TGC, log compression.
+struct DisplayArgs {
+  GridView g;
+  int apply_tgc;
+  float attenuation;   // dB / (cm MHz)
+  float f_mhz;         // TGC frequency, MHz
+  float dynamic_range_db;
+  float gain_db;
+};
+int display_blocks(const DisplayArgs& D, bool rf);
+// env (n_pix) = envelope (|IQ|, or the analytic magnitude along axis1 when rf)
+// times the TGC gain; blockmax[display_blocks] its block maxima.
+void launch_display_envelope(const DisplayArgs& D, const float2* frame, bool rf, float* env,
+                             float* blockmax, hipStream_t st);
+// image = log compression of env against *ref (device; <= 0: the maximum of blockmax).
+void launch_display_compress(const DisplayArgs& D, const float* env, const float* blockmax,
+                             int n_blocks, const float* ref, float* image, hipStream_t st);
+
 }  // namespace usxd
