@@ -53,7 +53,7 @@ def make_case(usx, O, n_el=32, fs=20e6, n_samples=768, n_tx=3, half_angle=9.0,
 
 def bf_config(usx, combiner="DAS", cf=False, compounding="COHERENT", f_number=1.5,
               subarray=16, loading=1e-2, range_avg=2, window="HANN", weighting=True,
-              precision="FP32", refine=True):
+              precision="FP32", refine=False):
     b = usx.BeamformerConfig()
     b.f_number = f_number
     b.combiner = getattr(usx.Combiner, combiner)

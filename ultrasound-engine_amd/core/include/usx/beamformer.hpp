@@ -33,7 +33,7 @@ struct BeamformerConfig {
   // Additions without a reference counterpart; the defaults keep the
   // reference's arithmetic.
   MvPrecision mv_precision = MvPrecision::FP32;
-  bool mv_refine = true;  // one step of iterative refinement, FP64 residual
+  bool mv_refine = false;  // one step of iterative refinement, FP64 residual (not in the reference)
   // Launch-shape tunables (0 = default): transmit events per beamforming
   // launch (1..16) and persistent MV workgroups per CU; usx.autotune picks them.
   int tile_slots = 0;

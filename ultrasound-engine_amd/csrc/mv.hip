@@ -10,8 +10,24 @@
 //   coherence factor; otherwise L = clamp(subarray, 2, max(2, n/2)) and the
 //   pixel is appended to the list of its size bucket P = 16 ceil(L/16).
 //
-// mv_solve<P> (persistent, one wavefront per pixel, LDS sized for the
-// bucket so small systems run at high occupancy):
+// mv_solve_fast<P, NR> (P <= 64, FP32, no refinement, no weight output: the
+// default path) -- one wavefront per pixel, persistent, the lower triangle
+// packed in LDS so up to 7 pixels of the largest bucket are resident per CU:
+//   1. staging as below, every depth's loads in flight together;
+//   2. covariance by the subarray shift recursion: lane m forms R(m,0)
+//      directly and walks the m-th sub-diagonal,
+//        R(i+1,k+1) = R(i,k) + sum_r x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k]),
+//      O(L n_snap + L^2 nrow) products instead of O(L^2 n_snap);
+//   3. loading as below;
+//   4. left-looking Cholesky in blocks of 8 columns, the forward
+//      substitutions for 1 and for the subarray average folded into the
+//      column loop;
+//   5. out = (y1^H y2) / |y1|^2 * n with y1 = L^-1 1, y2 = L^-1 avg, which is
+//      (w^H avg) / sum w * n for w = R^-1 1 without the back substitution.
+//
+// mv_solve<P, PREC> (the general path: P up to 96, BF16/FP16 covariance
+// inputs, refinement, effective-weight output) -- persistent, one wavefront
+// per pixel, LDS sized for the bucket:
 //   1. active list; centre and +-range_avg neighbour vectors staged in LDS
 //      (neighbours clipped at the grid edge, beamformer.cpp:159-161);
 //   2. covariance R = sum over (depth, subarray start) of x x^H as a dense
@@ -22,9 +38,9 @@
 //      upper-triangle 16x16 blocks only;
 //   3. R /= n_snap; diagonal loading max(delta tr(R)/L, 1e-20) with the trace
 //      summed in FP64;
-//   4. in-place FP32 Cholesky in LDS, element-parallel right-looking updates
-//      (the strictly upper triangle keeps R for the residual); a non-positive
-//      pivot -> DAS sum (beamformer.cpp:199);
+//   4. in-place FP32 Cholesky in LDS, left-looking by columns (the strictly
+//      upper triangle keeps R for the residual); a non-positive pivot -> DAS
+//      sum (beamformer.cpp:199);
 //   5. triangular solves for R w = 1 with the vector in registers (lane i
 //      holds w_i) and the pivot broadcast by readlane: no barriers;
 //   6. optional single refinement step: r = 1 - R w in FP64, R d = r, w += d;
@@ -107,6 +123,112 @@ __device__ __forceinline__ void store_out(const MvArgs& M, int pix, float2 out, 
   }
 }
 
+// Sum of the active centre vector (the DAS fallback).
+__device__ __forceinline__ float2 das_sum(const float2* yc, const int* act, int na, int lane) {
+  float sr = 0.0f, si = 0.0f;
+  for (int a = lane; a < na; a += 64) {
+    sr += yc[act[a]].x;
+    si += yc[act[a]].y;
+  }
+  return make_float2(wsum(sr), wsum(si));
+}
+
+struct Staged {
+  int na;    // active channels
+  int nrow;  // depths staged (centre and in-grid neighbours)
+  int crow;  // row index of the centre depth
+};
+
+// Steps 1-2 of a pixel: the active channel list act[0..na) and the centre and
+// +-range_avg neighbour vectors over those channels, rows[r * na + a]
+// (neighbours clipped at the grid edge, beamformer.cpp:159-161).
+__device__ __forceinline__ Staged stage_pixel(const MvArgs& M, const float2* __restrict__ Y,
+                                              const uint32_t* __restrict__ mask, int pix, int lane,
+                                              int* act, float2* rows) {
+  const int n_ch = M.n_ch;
+  const int nw = (n_ch + 31) / 32;
+  const int i0 = pix / M.n1;
+  const int j = pix - i0 * M.n1;
+  const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
+  int na = 0;
+  for (int base = 0; base < n_ch; base += 64) {
+    const int ch = base + lane;
+    const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
+    const unsigned long long bal = __ballot(on);
+    if (on) act[na + __popcll(bal & ((1ull << lane) - 1ull))] = ch;
+    na += __popcll(bal);
+  }
+  __syncthreads();
+  int nrow = 0, crow = 0;
+  for (int d = -M.range_avg; d <= M.range_avg; ++d) {
+    const int k = j + d;
+    if (k < 0 || k >= M.n1) continue;
+    if (d == 0) crow = nrow;
+    const float2* src = Y + (static_cast<size_t>(i0) * M.n1 + k) * n_ch;
+    // A channel inactive at the neighbouring depth contributes 0 there
+    // (its Y was never accumulated, beamformer.cpp:469); the channel
+    // kernel does not write such entries, so test the neighbour's mask.
+    const uint32_t* mk = mask + (static_cast<size_t>(i0) * M.n1 + k) * nw;
+    for (int a = lane; a < na; a += 64) {
+      const int ch = act[a];
+      const bool on = d == 0 || ((mk[ch >> 5] >> (ch & 31)) & 1u);
+      rows[nrow * na + a] = on ? src[ch] : make_float2(0.0f, 0.0f);
+    }
+    ++nrow;
+  }
+  __syncthreads();
+  return Staged{na, nrow, crow};
+}
+
+// stage_pixel with the depth count fixed at NR = 2 range_avg + 1: every
+// row's mask word and sample for a channel are loaded before any is stored,
+// so the NR global loads of a lane are in flight together. Rows past the
+// grid edge are zero (rows nrow .. NR-1).
+template <int NR>
+__device__ __forceinline__ Staged stage_pixel_nr(const MvArgs& M, const float2* __restrict__ Y,
+                                                 const uint32_t* __restrict__ mask, int pix,
+                                                 int lane, int* act, float2* rows) {
+  constexpr int RA = NR / 2;
+  const int n_ch = M.n_ch;
+  const int nw = (n_ch + 31) / 32;
+  const int i0 = pix / M.n1;
+  const int j = pix - i0 * M.n1;
+  const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
+  int na = 0;
+  for (int base = 0; base < n_ch; base += 64) {
+    const int ch = base + lane;
+    const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
+    const unsigned long long bal = __ballot(on);
+    if (on) act[na + __popcll(bal & ((1ull << lane) - 1ull))] = ch;
+    na += __popcll(bal);
+  }
+  __syncthreads();
+  const int dlo = max(-RA, -j), dhi = min(RA, M.n1 - 1 - j);  // in-grid depths j + d
+  const size_t line = static_cast<size_t>(i0) * M.n1 + j;
+#pragma unroll 2
+  for (int a = lane; a < na; a += 64) {
+    const int ch = act[a];
+    float2 v[NR];
+    uint32_t mw[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const size_t p = line + min(dlo + r, dhi);
+      v[r] = Y[p * n_ch + ch];
+      mw[r] = mask[p * nw + (ch >> 5)];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int d = dlo + r;
+      // a channel inactive at the neighbouring depth contributes 0 there
+      // (beamformer.cpp:469); the centre's channels are active by definition
+      const bool on = d <= dhi && (d == 0 || ((mw[r] >> (ch & 31)) & 1u));
+      rows[r * na + a] = on ? v[r] : make_float2(0.0f, 0.0f);
+    }
+  }
+  __syncthreads();
+  return Staged{na, dhi - dlo + 1, -dlo};
+}
+
 }  // namespace
 
 // Bytes of dynamic LDS for one wavefront of bucket P.
@@ -115,6 +237,15 @@ size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
   return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max +
                                   static_cast<size_t>(P) * (P + 1) + 2 * static_cast<size_t>(P));
+}
+
+// Bytes of dynamic LDS for mv_solve_fast: the packed lower triangle, no
+// diagonal or scratch vectors.
+size_t mv_fast_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
+  const int na_max = subarray <= P ? n_ch : std::min(n_ch, 2 * P + 1);
+  const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
+  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max +
+                                  static_cast<size_t>(P) * (P + 1) / 2);
 }
 
 __global__ void mv_zero_counts(int* counts) {
@@ -217,44 +348,14 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
   const int count = counts[BUCKET];
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const int pix = lists[static_cast<size_t>(BUCKET) * n_pix + item];
-    const int i0 = pix / M.n1;
-    const int j = pix - i0 * M.n1;
     __syncthreads();  // LDS of the previous pixel is free
 
-    // ---- 1. active list ------------------------------------------------------
-    const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
-    int na = 0;
-    for (int base = 0; base < n_ch; base += 64) {
-      const int ch = base + lane;
-      const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
-      const unsigned long long bal = __ballot(on);
-      if (on) act[na + __popcll(bal & ((1ull << lane) - 1ull))] = ch;
-      na += __popcll(bal);
-    }
-    __syncthreads();
+    // ---- 1-2. active list, centre and neighbour rows ------------------------
+    const Staged sg = stage_pixel(M, Y, mask, pix, lane, act, rows);
+    const int na = sg.na, nrow = sg.nrow, crow = sg.crow;
     const float2* yc = Y + static_cast<size_t>(pix) * n_ch;
     const int L = min(max(M.subarray, 2), max(2, na / 2));
     const int ns = na - L + 1;
-
-    // ---- 2. stage centre and neighbour rows ---------------------------------
-    int nrow = 0, crow = 0;
-    for (int d = -M.range_avg; d <= M.range_avg; ++d) {
-      const int k = j + d;
-      if (k < 0 || k >= M.n1) continue;
-      if (d == 0) crow = nrow;
-      const float2* src = Y + (static_cast<size_t>(i0) * M.n1 + k) * n_ch;
-      // A channel inactive at the neighbouring depth contributes 0 there
-      // (its Y was never accumulated, beamformer.cpp:469); the channel
-      // kernel does not write such entries, so test the neighbour's mask.
-      const uint32_t* mk = mask + (static_cast<size_t>(i0) * M.n1 + k) * nw;
-      for (int a = lane; a < na; a += 64) {
-        const int ch = act[a];
-        const bool on = d == 0 || ((mk[ch >> 5] >> (ch & 31)) & 1u);
-        rows[nrow * na + a] = on ? src[ch] : make_float2(0.0f, 0.0f);
-      }
-      ++nrow;
-    }
-    __syncthreads();
     const int n_snap = nrow * ns;
 
     // ---- 3. covariance on MFMA ----------------------------------------------
@@ -592,30 +693,292 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
   }
 }
 
+// The FP32 path without refinement or weight output, for P <= 64 (one matrix
+// row per lane). Same staging, loading and fallbacks as mv_solve; three
+// changes of formulation, each exact in real arithmetic:
+//
+// * Covariance by the subarray shift recursion. With
+//     R(i,k) = sum_r sum_{a<n_sub} x_r[a+i] conj(x_r[a+k]),
+//   R(i+1,k+1) = R(i,k) + sum_r (x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k])),
+//   so lane m forms the first-column entry R(m,0) directly (n_snap products)
+//   and walks the m-th sub-diagonal adding 2 nrow products per step:
+//   O(L n_snap + L^2 nrow) per pixel instead of O(L^2 n_snap).
+// * The forward substitutions run inside the Cholesky column loop
+//   (column-oriented: once column j of the factor is known, y_j = b_j / L_jj
+//   and b_i -= L_ij y_j below it), for b = 1 and b = avg.
+// * No back substitution: with w = R^-1 1, sum w = 1^H R^-1 1 = |y1|^2 and
+//   w^H avg = y1^H y2 (y1 = L^-1 1, y2 = L^-1 avg), so
+//   out = (y1^H y2) / |y1|^2 * n.
+template <int P, int NR>
+__global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
+                                                    const uint32_t* __restrict__ mask,
+                                                    float2* __restrict__ frame, float* acc,
+                                                    const int* __restrict__ counts,
+                                                    const int* __restrict__ lists, int na_max) {
+  static_assert(P <= 64, "one matrix row per lane");
+  constexpr int BUCKET = P / 16 - 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x;
+  const int n_ch = M.n_ch;
+  const int n_pix = M.n0 * M.n1;
+  int* act = reinterpret_cast<int*>(smem_raw);
+  float2* rows = reinterpret_cast<float2*>(smem_raw + (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16);
+  // lower triangle, packed by rows: (i, k <= i) at i (i + 1) / 2 + k
+  float2* A = rows + static_cast<size_t>(NR) * na_max;
+  auto tri = [](int i) { return (i * (i + 1)) >> 1; };
+
+  const int count = counts[BUCKET];
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const int pix = lists[static_cast<size_t>(BUCKET) * n_pix + item];
+    __syncthreads();  // LDS of the previous pixel is free
+    const Staged sg = stage_pixel_nr<NR>(M, Y, mask, pix, lane, act, rows);
+    const int na = sg.na, nrow = sg.nrow;
+    const float2* yc = Y + static_cast<size_t>(pix) * n_ch;
+    const int L = min(max(M.subarray, 2), max(2, na / 2));
+    const int ns = na - L + 1;
+    const float inv = 1.0f / static_cast<float>(nrow * ns);
+
+    // ---- covariance: first column, then the sub-diagonal walks ---------------
+    // Loops have uniform bounds and fixed row counts (NR rows, the missing
+    // ones zero) so the loads of a step are all issued before its products;
+    // lanes past their diagonal's end read clamped indices and store nothing.
+    double tr = 0.0;  // lane 0 walks the diagonal and sums the trace in FP64
+    float cr, ci;
+    {
+      float ar[2] = {0.0f, 0.0f}, ai[2] = {0.0f, 0.0f};
+      const int m = min(lane, L - 1);
+      for (int a = 0; a < ns; a += 4) {  // R(m,0) += x[a+m] conj(x[a])
+        float2 p[NR][4], q[NR][4];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const float2* x = rows + r * na;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int au = min(a + u, ns - 1);
+            p[r][u] = x[au + m];
+            q[r][u] = a + u < ns ? x[au] : make_float2(0.0f, 0.0f);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float2 pp = p[r][u], qq = q[r][u];
+            ar[u & 1] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u & 1]));
+            ai[u & 1] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u & 1]));
+          }
+        }
+      }
+      cr = ar[0] + ar[1];
+      ci = ai[0] + ai[1];
+    }
+    if (lane < L) A[tri(lane)] = make_float2(cr * inv, lane == 0 ? 0.0f : ci * inv);
+    if (lane == 0) tr = static_cast<double>(cr * inv);
+#pragma unroll 2
+    for (int t = 0; t < L - 1; ++t) {
+      // R(lane+t+1, t+1) = R(lane+t, t) + sum_r (hi conj(hk) - lo conj(lk))
+      const int il = min(lane + t, L - 2);
+      float2 hi[NR], hk[NR], lo[NR], lk[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const float2* x = rows + r * na;
+        hi[r] = x[il + ns];
+        hk[r] = x[t + ns];
+        lo[r] = x[il];
+        lk[r] = x[t];
+      }
+      float dr = 0.0f, di = 0.0f;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        dr = fmaf(hi[r].x, hk[r].x, fmaf(hi[r].y, hk[r].y, dr));
+        di = fmaf(hi[r].y, hk[r].x, fmaf(-hi[r].x, hk[r].y, di));
+        dr = fmaf(-lo[r].x, lk[r].x, fmaf(-lo[r].y, lk[r].y, dr));
+        di = fmaf(-lo[r].y, lk[r].x, fmaf(lo[r].x, lk[r].y, di));
+      }
+      cr += dr;
+      ci += di;
+      const float vr = cr * inv;
+      if (lane + t + 1 < L) A[tri(lane + t + 1) + t + 1] = make_float2(vr, lane == 0 ? 0.0f : ci * inv);
+      if (lane == 0) tr += static_cast<double>(vr);
+    }
+    // ---- diagonal loading max(delta tr / L, 1e-20) ---------------------------
+    tr = __shfl(tr, 0, 64);
+    const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
+    __syncthreads();
+    // subarray average of the centre vector: the second right-hand side
+    float b2r = 0.0f, b2i = 0.0f;
+    if (lane < L) {
+      A[tri(lane) + lane].x += eps;
+      const float2* cen = rows + sg.crow * na + lane;
+      float pr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int s0 = 0; s0 < ns; s0 += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float2 x = cen[min(s0 + u, ns - 1)];
+          v[u] = s0 + u < ns ? x : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          pr[u & 3] += v[u].x;
+          pi[u & 3] += v[u].y;
+        }
+      }
+      b2r = ((pr[0] + pr[1]) + (pr[2] + pr[3])) / static_cast<float>(ns);
+      b2i = ((pi[0] + pi[1]) + (pi[2] + pi[3])) / static_cast<float>(ns);
+    }
+    __syncthreads();
+
+    // ---- Cholesky, left-looking in blocks of CB columns, with both forward
+    // substitutions. For the block [j0, j0 + CB) lane i first forms
+    //   s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k))
+    // (one read of its own row per k feeds CB accumulators), then finishes the
+    // block column by column with the in-block terms k = j0 .. j0+c-1 (taken
+    // by readlane from the registers of lane j0+c), so every sum runs over k
+    // in increasing order as in the reference (beamformer.cpp:48-63).
+    constexpr int CB = 8;
+    float b1r = lane < L ? 1.0f : 0.0f, b1i = 0.0f;
+    bool ok = true;
+    for (int j0 = 0; j0 < L && ok; j0 += CB) {
+      float sr[CB], si[CB], lr[CB], li[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        sr[c] = 0.0f;
+        si[c] = 0.0f;
+        lr[c] = 0.0f;
+        li[c] = 0.0f;
+      }
+      if (lane >= j0 && lane < L) {
+        const float2* ri_ = A + tri(lane);
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          if (j0 + c <= lane) {
+            sr[c] = ri_[j0 + c].x;
+            si[c] = ri_[j0 + c].y;
+          }
+        }
+#pragma unroll 2
+        for (int k = 0; k < j0; ++k) {
+          const float2 a = ri_[k];
+#pragma unroll
+          for (int c = 0; c < CB; ++c) {
+            const float2 b = A[tri(min(j0 + c, L - 1)) + k];  // -= a conj(b)
+            sr[c] = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr[c]));
+            si[c] = fmaf(-a.y, b.x, fmaf(a.x, b.y, si[c]));
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        const int j = j0 + c;
+        if (j >= L) break;
+#pragma unroll
+        for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
+          const float br = lane_bcast(lr[q], j), bi = lane_bcast(li[q], j);
+          sr[c] = fmaf(-lr[q], br, fmaf(-li[q], bi, sr[c]));
+          si[c] = fmaf(-li[q], br, fmaf(lr[q], bi, si[c]));
+        }
+        const float d = lane_bcast(sr[c], j);
+        if (!(d > 0.0f)) {
+          ok = false;
+          break;
+        }
+        // 1 / L_jj in one instruction (v_rsq_f32, ~1 ulp); L_jj itself is
+        // only stored, never read back.
+        const float rj = __builtin_amdgcn_rsqf(d);
+        const float y1r = lane_bcast(b1r, j) * rj, y1i = lane_bcast(b1i, j) * rj;
+        const float y2r = lane_bcast(b2r, j) * rj, y2i = lane_bcast(b2i, j) * rj;
+        if (lane == j) {
+          lr[c] = d * rj;
+          li[c] = 0.0f;
+          b1r = y1r;
+          b1i = y1i;
+          b2r = y2r;
+          b2i = y2i;
+        } else if (lane > j && lane < L) {
+          lr[c] = sr[c] * rj;
+          li[c] = si[c] * rj;
+          b1r = fmaf(-lr[c], y1r, fmaf(li[c], y1i, b1r));  // b -= l y
+          b1i = fmaf(-lr[c], y1i, fmaf(-li[c], y1r, b1i));
+          b2r = fmaf(-lr[c], y2r, fmaf(li[c], y2i, b2r));
+          b2i = fmaf(-lr[c], y2i, fmaf(-li[c], y2r, b2i));
+        }
+      }
+      if (lane >= j0 && lane < L) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+          if (j0 + c < L && j0 + c <= lane) A[tri(lane) + j0 + c] = make_float2(lr[c], li[c]);
+      }
+      __syncthreads();
+    }
+    float2 out;
+    bool das = !ok;
+    if (ok) {
+      const bool on = lane < L;
+      const float dn = wsum(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);       // |y1|^2 = sum w
+      const float nr = wsum(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);       // y1^H y2 = w^H avg
+      const float ni = wsum(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
+      if (dn < 1e-30f) {
+        das = true;
+      } else {
+        const float nf = static_cast<float>(na);
+        out = make_float2(nr / dn * nf, ni / dn * nf);
+      }
+    }
+    if (das) out = das_sum(yc, act, na, lane);
+    if (M.status && lane == 0 && das) M.status[pix] = 2;
+    if (M.cf) {
+      const float f = coherence(yc, act, na, lane);
+      out.x *= f;
+      out.y *= f;
+    }
+    if (lane == 0) store_out(M, pix, out, frame, acc);
+  }
+}
+
 template <int P>
 static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
                           float* acc, const int* counts, const int* lists, hipStream_t st) {
-  const size_t smem = mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
+  const bool fast = P <= 64 && M.precision == 0 && !M.refine && M.weights == nullptr &&
+                    M.range_avg >= 0 && M.range_avg <= 2;
+  const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
+                           : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
   const int n_pix = M.n0 * M.n1;
-  // persistent grid: as many single-wavefront workgroups per CU as LDS allows
-  // (capped at 32, or at the tuned occupancy)
+  // Persistent grid: as many single-wavefront workgroups per CU as can be
+  // resident at once -- bounded by LDS (160 KB / smem), by VGPRs (4 SIMDs x
+  // min(8, 512 / VGPRs per wave), read from the kernel's attributes once)
+  // and by the tuned occupancy when one is given.
   const int cap = M.occupancy > 0 ? M.occupancy : 32;
-  const int per_cu = std::max(1, std::min(cap, static_cast<int>(160 * 1024 / smem)));
-  const int grid = std::min(n_pix, 256 * per_cu);
-#define USX_MV(PR)                                                                            \
-  do {                                                                                        \
-    static const hipError_t attr = hipFuncSetAttribute(                                       \
-        reinterpret_cast<const void*>(mv_solve<P, PR>),                                       \
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                              \
-    (void)attr;                                                                               \
-    hipLaunchKernelGGL((mv_solve<P, PR>), dim3(grid), dim3(64), smem, st, M, Y, mask, frame,  \
-                       acc, counts, lists, na_max);                                           \
+  const int lds_cap = static_cast<int>(160 * 1024 / smem);
+#define USX_LAUNCH(KERNEL)                                                                        \
+  do {                                                                                           \
+    static const int vgpr_cap = [] {                                                             \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(KERNEL),                           \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);         \
+      hipFuncAttributes fa{};                                                                    \
+      if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(KERNEL)) != hipSuccess ||      \
+          fa.numRegs <= 0)                                                                       \
+        return 16;                                                                               \
+      return 4 * std::max(1, std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8)));                    \
+    }();                                                                                         \
+    const int per_cu = std::max(1, std::min(std::min(cap, lds_cap), vgpr_cap));                  \
+    const int grid = std::min(n_pix, 256 * per_cu);                                              \
+    hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(64), smem, st, M, Y, mask, frame, acc, counts,   \
+                       lists, na_max);                                                           \
   } while (0)
-  if (M.precision == 1) USX_MV(1);
-  else if (M.precision == 2) USX_MV(2);
-  else USX_MV(0);
-#undef USX_MV
+  if constexpr (P <= 64) {
+    if (fast) {
+      if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1>));
+      else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3>));
+      else USX_LAUNCH((mv_solve_fast<P, 5>));
+      return;
+    }
+  }
+  if (M.precision == 1) USX_LAUNCH((mv_solve<P, 1>));
+  else if (M.precision == 2) USX_LAUNCH((mv_solve<P, 2>));
+  else USX_LAUNCH((mv_solve<P, 0>));
+#undef USX_LAUNCH
 }
 
 int mv_max_bucket_p(int n_ch, int subarray) {
