@@ -46,6 +46,9 @@ def test_learned_contrast_matches_mv(result):
 
 
 def test_learned_frame_is_much_cheaper_than_mv(usx_mod):
+    """Device-resident frame time (RF already in HBM, events on the library's
+    stream): the learned frame costs well under a quarter of minimum
+    variance's (bench.py reports the measured ratio)."""
     from usx import learned as L
     probe, med, acq, sim, txs = L.acquisition_setup()
     grid = usx_mod.grid_for(probe, 0.005, 0.050, 256, 640)
@@ -57,14 +60,8 @@ def test_learned_frame_is_much_cheaper_than_mv(usx_mod):
     for name in ("MV", "Learned"):
         pc = L.mv_config()
         pc.beamformer.combiner = getattr(usx_mod.Combiner, name)
-        pipe = usx_mod.ImagingPipeline(grid, pc)
+        runner = usx_mod.DeviceRunner(grid, pc, raw)
         if name == "Learned":
-            pipe.set_apodization(params)
-        pipe.process_frame(raw)
-        t = time.perf_counter()
-        for _ in range(5):
-            pipe.process_frame(raw)
-        times[name] = (time.perf_counter() - t) / 5
-    # end to end (host RF in, host frame out): the learned frame is >= 3x
-    # faster even with the PCIe transfers that both pay
-    assert times["Learned"] * 3 < times["MV"], times
+            runner.set_apodization(params)
+        times[name] = runner.time_host(raw, 10)
+    assert times["Learned"] * 4 < times["MV"], times

@@ -177,6 +177,43 @@ class DeviceRunner {
     }
   }
 
+  // Device-resident ms per frame on host channel data of the template's
+  // shape: uploaded once into temporary device buffers, one untimed frame,
+  // then `reps` frames between two events on the context's stream.
+  float time_host(const ChannelData& data, int reps) {
+    if (!is_rf_) throw std::invalid_argument("time_host: the runner's template must be RF");
+    if (data.rf.size() != static_cast<std::size_t>(dplan_.desc.n_traces) * dplan_.desc.n_in)
+      throw std::invalid_argument("channel data does not match the runner's template");
+    if (reps < 1) throw std::invalid_argument("time_host: reps must be >= 1");
+    usx_ctx* c = ctx_.get();
+    void* rf = nullptr;
+    void* fr = nullptr;
+    const std::size_t nf = static_cast<std::size_t>(bplan_.desc.n_axis0) * bplan_.desc.n_axis1;
+    check(c, usx_buffer_alloc(c, data.rf.size() * sizeof(float), &rf), "buffer_alloc");
+    const usx_status st = usx_buffer_alloc(c, nf * sizeof(usx_c32), &fr);
+    if (st != USX_OK) {
+      usx_buffer_free(c, rf);
+      check(c, st, "buffer_alloc");
+    }
+    try {
+      check(c, usx_buffer_upload(c, rf, data.rf.data(), data.rf.size() * sizeof(float)), "buffer_upload");
+      const auto in = reinterpret_cast<std::uintptr_t>(rf), out = reinterpret_cast<std::uintptr_t>(fr);
+      run(in, out);
+      record(0);
+      for (int i = 0; i < reps; ++i) run(in, out);
+      record(1);
+      synchronize();
+      const float ms = elapsed(0, 1) / static_cast<float>(reps);
+      usx_buffer_free(c, rf);
+      usx_buffer_free(c, fr);
+      return ms;
+    } catch (...) {
+      usx_buffer_free(c, rf);
+      usx_buffer_free(c, fr);
+      throw;
+    }
+  }
+
   // ---- learned apodization (no reference counterpart) ----------------------
   using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
 
@@ -961,6 +998,8 @@ PYBIND11_MODULE(_usx, m) {
            py::arg("sw"))
       .def("autotune", &DeviceRunner::autotune, py::arg("input_ptr"), py::arg("frame_ptr"),
            py::arg("reps") = 3)
+      .def("time_host", &DeviceRunner::time_host, py::arg("channel_data"), py::arg("reps") = 10,
+           py::call_guard<py::gil_scoped_release>())
       .def("autotune_host", &DeviceRunner::autotune_host, py::arg("channel_data"),
            py::arg("reps") = 3);
 

@@ -43,7 +43,10 @@ constexpr int kL = kApodLayers;      // 4
 constexpr int kMT = kF / 16;         // 16 output tiles
 constexpr int kKB = kF / 32;         // 8 K blocks
 constexpr int kLayerBf16 = kF * kF;  // swizzled bf16 per layer
-constexpr int kNT = 1;               // 16-sample tiles per wavefront
+constexpr int kNT = 1;               // 16-sample tiles per wavefront (training)
+constexpr int kNTi = 1;              // 16-sample tiles per wavefront (inference)
+constexpr int kInferWaves = 4;       // wavefronts per inference workgroup
+constexpr int kStageTiles = 4;       // output tiles per LDS stage (of kMT per layer)
 
 // FP32 -> BF16, round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint32_t f2bf(float x) {
@@ -83,18 +86,19 @@ __host__ __device__ __forceinline__ int frag_pos(int row, int k) {
   return ((m * kKB + kb) * 64 + g * 16 + r) * 8 + j;
 }
 
-// One layer on a wavefront's kNT sample tiles: acc[m][t] = bias + A_m . act[.][t]
+// One layer on a wavefront's NT sample tiles: acc[m][t] = bias + A_m . act[.][t]
 // with A fragments from `w` (LDS or global, 16 bytes per lane per fragment).
 // The fragments of tile m + 1 are loaded while tile m's MFMAs run; the
 // scheduling barrier keeps the compiler from hoisting every load of the
 // layer (128 fragments) to the top, which would need 512 VGPRs.
-template <bool kGlobal>
-__device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (&act)[kKB][kNT],
-                                      f4 (&acc)[kMT][kNT], int lane) {
+template <bool kGlobal, int NT = kNT, int M0 = 0, int M1 = kMT>
+__device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (&act)[kKB][NT],
+                                      f4 (&acc)[kMT][NT], int lane) {
   const int g = lane >> 4;
   u4 cur[kKB], nxt[kKB];
+  // w holds the fragments of tiles M0 .. M1-1
   auto load = [&](int m, u4 (&dst)[kKB]) {
-    const u4* wm = w + m * kKB * 64;
+    const u4* wm = w + (m - M0) * kKB * 64;
     // Global fragments: an opaque per-tile uniform base, so the compiler
     // addresses each load as SGPR base + lane offset instead of hoisting
     // 128 per-lane 64-bit addresses out of the layer loop.
@@ -102,21 +106,21 @@ __device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (
 #pragma unroll
     for (int kb = 0; kb < kKB; ++kb) dst[kb] = wm[kb * 64 + lane];
   };
-  load(0, cur);
+  load(M0, cur);
 #pragma unroll
-  for (int m = 0; m < kMT; ++m) {
-    if (m + 1 < kMT) load(m + 1, nxt);
+  for (int m = M0; m < M1; ++m) {
+    if (m + 1 < M1) load(m + 1, nxt);
     f4 b = f4{0.0f, 0.0f, 0.0f, 0.0f};
     if (bias) {
       const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * m + 4 * g);
       b = f4{bb.x, bb.y, bb.z, bb.w};
     }
 #pragma unroll
-    for (int t = 0; t < kNT; ++t) acc[m][t] = b;
+    for (int t = 0; t < NT; ++t) acc[m][t] = b;
 #pragma unroll
     for (int kb = 0; kb < kKB; ++kb)
 #pragma unroll
-      for (int t = 0; t < kNT; ++t)
+      for (int t = 0; t < NT; ++t)
         acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(cur[kb]), as_bf8(act[kb][t]),
                                                             acc[m][t], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
@@ -189,11 +193,12 @@ __device__ __forceinline__ int load_features(const float2* Y, const uint32_t* ma
 }
 
 // The next layer's B operand from output tiles 2 kb and 2 kb + 1, through GELU.
-__device__ __forceinline__ void activate(const f4 (&acc)[kMT][kNT], u4 (&act)[kKB][kNT]) {
+template <int NT = kNT>
+__device__ __forceinline__ void activate(const f4 (&acc)[kMT][NT], u4 (&act)[kKB][NT]) {
 #pragma unroll
   for (int kb = 0; kb < kKB; ++kb)
 #pragma unroll
-    for (int t = 0; t < kNT; ++t) {
+    for (int t = 0; t < NT; ++t) {
       const f4 a = acc[2 * kb][t], b = acc[2 * kb + 1][t];
       act[kb][t] = u4{pack2(gelu(a[0]), gelu(a[1])), pack2(gelu(a[2]), gelu(a[3])),
                       pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
@@ -204,11 +209,47 @@ __device__ __forceinline__ void activate(const f4 (&acc)[kMT][kNT], u4 (&act)[kK
 
 }  // namespace
 
+// Weights of the inference kernel move HBM/L2 -> registers -> LDS one stage
+// (kStageTiles output tiles of a layer) at a time; each thread carries
+// kPre 16-byte rows of the next stage in registers while the current stage's
+// MFMAs run, so the global loads overlap the compute.
+constexpr int kStageU4 = kStageTiles * kKB * 64;
+constexpr int kPre = kStageU4 / (64 * kInferWaves);
+static_assert(kMT % kStageTiles == 0 && kStageU4 % (64 * kInferWaves) == 0, "stage shape");
+
+__device__ __forceinline__ void fetch_stage(const u4* wsrc, int l, int m0, int tid, u4 (&pre)[kPre]) {
+  const u4* src = wsrc + l * (kLayerBf16 / 8) + m0 * kKB * 64;
+#pragma unroll
+  for (int q = 0; q < kPre; ++q) pre[q] = src[tid + q * 64 * kInferWaves];
+}
+
+// Layer l from output tile M0 on: store the prefetched stage, start fetching
+// the next one (this layer's next stage, else layer l + 1's first), run this
+// stage's tiles; compile-time recursion over the stages so the accumulator
+// tiles stay register-indexed.
+template <int M0>
+__device__ __forceinline__ void stage_and_run(u4* wl, const u4* wsrc, int l, const float* bias,
+                                              const u4 (&act)[kKB][kNTi], f4 (&acc)[kMT][kNTi],
+                                              u4 (&pre)[kPre], int tid, int lane) {
+  if constexpr (M0 < kMT) {
+    __syncthreads();  // every wavefront is done reading the previous stage
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) wl[tid + q * 64 * kInferWaves] = pre[q];
+    __syncthreads();
+    if (M0 + kStageTiles < kMT) fetch_stage(wsrc, l, M0 + kStageTiles, tid, pre);
+    else if (l + 1 < kL) fetch_stage(wsrc, l + 1, 0, tid, pre);
+    layer<false, kNTi, M0, M0 + kStageTiles>(wl, bias, act, acc, lane);
+    stage_and_run<M0 + kStageTiles>(wl, wsrc, l, bias, act, acc, pre, tid, lane);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Inference: one workgroup = 8 wavefronts x 32 pixels; each layer's 128 KB
-// of swizzled weights is staged in LDS and shared by the 8 wavefronts.
+// Inference: one workgroup = kInferWaves wavefronts x 16 kNTi pixels. Each
+// layer's swizzled weights pass through LDS kStageTiles output tiles at a
+// time (32 KB for 4 of the 16 tiles), shared by the workgroup's wavefronts;
+// the next stage is fetched into registers while the current one computes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
+__global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
                                                          const float2* __restrict__ Y,
                                                          const uint32_t* __restrict__ mask,
                                                          float2* __restrict__ frame, float* acc_out,
@@ -216,31 +257,30 @@ __global__ __launch_bounds__(512) void apod_infer_kernel(ApodNet N, int n_pix, i
   extern __shared__ u4 wl[];  // kLayerBf16 bf16
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const long base = (static_cast<long>(blockIdx.x) * 8 + wv) * (16 * kNT);
-  u4 act[kKB][kNT];
-  int na[kNT];
+  const long base = (static_cast<long>(blockIdx.x) * kInferWaves + wv) * (16 * kNTi);
+  u4 act[kKB][kNTi];
+  int na[kNTi];
 #pragma unroll
-  for (int t = 0; t < kNT; ++t) {
+  for (int t = 0; t < kNTi; ++t) {
     const long pix = base + 16 * t + r;
     u4 x[kKB];
     na[t] = load_features(Y, mask, n_ch, pix, pix < n_pix, lane, x);
 #pragma unroll
     for (int kb = 0; kb < kKB; ++kb) act[kb][t] = x[kb];
   }
-  f4 acc[kMT][kNT];
+  f4 acc[kMT][kNTi];
   const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
+  u4 pre[kPre];
+  fetch_stage(wsrc, 0, 0, tid, pre);
   for (int l = 0; l < kL; ++l) {
-    __syncthreads();
-    for (int q = tid; q < kLayerBf16 / 8; q += 512) wl[q] = wsrc[l * (kLayerBf16 / 8) + q];
-    __syncthreads();
-    layer<false>(wl, N.bias + l * kF, act, acc, lane);
-    if (l + 1 < kL) activate(acc, act);
+    stage_and_run<0>(wl, wsrc, l, N.bias + l * kF, act, acc, pre, tid, lane);
+    if (l + 1 < kL) activate<kNTi>(acc, act);
   }
   // out = sum over active channels of (1 + o_c) y_c; fewer than 4 active
   // channels -> the DAS sum, as minimum variance does.
   const int nw = (n_ch + 31) / 32;
 #pragma unroll
-  for (int t = 0; t < kNT; ++t) {
+  for (int t = 0; t < kNTi; ++t) {
     const long pix = base + 16 * t + r;
     const bool valid = pix < n_pix;
     uint32_t mw[4] = {0u, 0u, 0u, 0u};
@@ -630,7 +670,7 @@ __global__ __launch_bounds__(256) void apod_pack_kernel(ApodNet N) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-size_t apod_infer_smem() { return sizeof(uint16_t) * kLayerBf16; }
+size_t apod_infer_smem() { return sizeof(u4) * kStageU4; }
 
 void launch_apod_pack(const ApodNet& N, hipStream_t st) {
   hipLaunchKernelGGL(apod_pack_kernel, dim3((kApodParams + 255) / 256), dim3(256), 0, st, N);
@@ -639,12 +679,12 @@ void launch_apod_pack(const ApodNet& N, hipStream_t st) {
 void launch_apod_infer(const ApodNet& N, int n_pix, int n_ch, const float2* Y,
                        const uint32_t* mask, float2* frame, float* acc, int cf, int incoherent,
                        hipStream_t st) {
-  const int per_block = 8 * 16 * kNT;
+  const int per_block = kInferWaves * 16 * kNTi;
   static const hipError_t attr = hipFuncSetAttribute(
       reinterpret_cast<const void*>(apod_infer_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
       static_cast<int>(apod_infer_smem()));
   (void)attr;
-  hipLaunchKernelGGL(apod_infer_kernel, dim3((n_pix + per_block - 1) / per_block), dim3(512),
+  hipLaunchKernelGGL(apod_infer_kernel, dim3((n_pix + per_block - 1) / per_block), dim3(64 * kInferWaves),
                      apod_infer_smem(), st, N, n_pix, n_ch, Y, mask, frame, acc, cf, incoherent);
 }
 
