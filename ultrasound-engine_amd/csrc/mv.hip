@@ -19,7 +19,8 @@
 //        R(i+1,k+1) = R(i,k) + sum_r x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k]),
 //      O(L n_snap + L^2 nrow) products instead of O(L^2 n_snap);
 //   3. loading as below;
-//   4. left-looking Cholesky in blocks of 8 columns, the forward
+//   4. left-looking Cholesky in blocks of 8 columns on the lower triangle
+//      packed by columns, the forward
 //      substitutions for 1 and for the subarray average folded into the
 //      column loop;
 //   5. out = (y1^H y2) / |y1|^2 * n with y1 = L^-1 1, y2 = L^-1 avg, which is
@@ -180,25 +181,65 @@ __device__ __forceinline__ Staged stage_pixel(const MvArgs& M, const float2* __r
   return Staged{na, nrow, crow};
 }
 
-// stage_pixel with the depth count fixed at NR = 2 range_avg + 1: every
+// Lanes of a wavefront split into G groups of LP = 64 / G lanes, one pixel
+// per group. group_bcast: v of lane k of the caller's group (readlanes and
+// selects); group_sum: the sum over the caller's group.
+template <int G>
+__device__ __forceinline__ float group_bcast(float v, int k, int lane) {
+  constexpr int LP = 64 / G;
+  float r = lane_bcast(v, k);
+#pragma unroll
+  for (int g = 1; g < G; ++g) {
+    const float x = lane_bcast(v, g * LP + k);
+    r = lane >= g * LP ? x : r;
+  }
+  return r;
+}
+template <int G>
+__device__ __forceinline__ double group_bcast_d(double v, int k, int lane) {
+  const long long b = __double_as_longlong(v);
+  const float lo = group_bcast<G>(__int_as_float(static_cast<int>(b & 0xffffffffll)), k, lane);
+  const float hi = group_bcast<G>(__int_as_float(static_cast<int>(b >> 32)), k, lane);
+  return __longlong_as_double((static_cast<long long>(__float_as_int(hi)) << 32) |
+                              static_cast<unsigned int>(__float_as_int(lo)));
+}
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = 32 / G; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// stage_pixel for one pixel per lane group, with the depth count fixed at
+// NR = 2 range_avg + 1 and the row stride at RS >= n_ch (compile-time, so
+// the NR rows of one index are one address plus immediate offsets): every
 // row's mask word and sample for a channel are loaded before any is stored,
-// so the NR global loads of a lane are in flight together. Rows past the
-// grid edge are zero (rows nrow .. NR-1).
-template <int NR>
-__device__ __forceinline__ Staged stage_pixel_nr(const MvArgs& M, const float2* __restrict__ Y,
-                                                 const uint32_t* __restrict__ mask, int pix,
-                                                 int lane, int* act, float2* rows) {
+// so a lane's NR global loads are in flight together. Rows past the grid
+// edge are zero (rows nrow .. NR-1). act and rows are the group's own.
+template <int NR, int RS, int G>
+__device__ __forceinline__ Staged stage_group(const MvArgs& M, const float2* __restrict__ Y,
+                                              const uint32_t* __restrict__ mask, int pix, int lane,
+                                              int* act, float2* rows) {
   constexpr int RA = NR / 2;
+  constexpr int LP = 64 / G;
+  const int sub = lane % LP, g = lane / LP;
+  const unsigned long long gbits =
+      G == 1 ? ~0ull : (((1ull << LP) - 1ull) << (g * LP));
   const int n_ch = M.n_ch;
   const int nw = (n_ch + 31) / 32;
   const int i0 = pix / M.n1;
   const int j = pix - i0 * M.n1;
   const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
   int na = 0;
-  for (int base = 0; base < n_ch; base += 64) {
-    const int ch = base + lane;
+  for (int base = 0; base < n_ch; base += LP) {
+    const int ch = base + sub;
     const bool on = ch < n_ch && ((mp[ch >> 5] >> (ch & 31)) & 1u);
-    const unsigned long long bal = __ballot(on);
+    const unsigned long long bal = __ballot(on) & gbits;
     if (on) act[na + __popcll(bal & ((1ull << lane) - 1ull))] = ch;
     na += __popcll(bal);
   }
@@ -206,7 +247,7 @@ __device__ __forceinline__ Staged stage_pixel_nr(const MvArgs& M, const float2* 
   const int dlo = max(-RA, -j), dhi = min(RA, M.n1 - 1 - j);  // in-grid depths j + d
   const size_t line = static_cast<size_t>(i0) * M.n1 + j;
 #pragma unroll 2
-  for (int a = lane; a < na; a += 64) {
+  for (int a = sub; a < na; a += LP) {
     const int ch = act[a];
     float2 v[NR];
     uint32_t mw[NR];
@@ -222,7 +263,7 @@ __device__ __forceinline__ Staged stage_pixel_nr(const MvArgs& M, const float2* 
       // a channel inactive at the neighbouring depth contributes 0 there
       // (beamformer.cpp:469); the centre's channels are active by definition
       const bool on = d <= dhi && (d == 0 || ((mw[r] >> (ch & 31)) & 1u));
-      rows[r * na + a] = on ? v[r] : make_float2(0.0f, 0.0f);
+      rows[r * RS + a] = on ? v[r] : make_float2(0.0f, 0.0f);
     }
   }
   __syncthreads();
@@ -239,13 +280,27 @@ size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
                                   static_cast<size_t>(P) * (P + 1) + 2 * static_cast<size_t>(P));
 }
 
-// Bytes of dynamic LDS for mv_solve_fast: the packed lower triangle, no
-// diagonal or scratch vectors.
+// Row stride of mv_solve_fast's staged vectors (float2) for bucket P, or 0
+// when the channel count is beyond its instantiations. When the subarray
+// exceeds P, every pixel of the bucket has L = n / 2 <= P, so at most
+// 2P + 1 active channels: the rows take round8(2P + 1) instead of n_ch.
+constexpr int mv_tight_stride(int P) { return (2 * P + 1 + 7) / 8 * 8; }
+static int mv_fast_row_stride(int n_ch, int subarray, int P) {
+  if (subarray > P && mv_tight_stride(P) < 128) return mv_tight_stride(P);
+  return n_ch <= 128 ? 128 : (n_ch <= 256 ? 256 : 0);
+}
+
+// Pixels per wavefront of mv_solve_fast for bucket P: groups of 16, 32 or
+// 64 lanes, one matrix row per lane.
+static int mv_fast_groups(int P) { return P <= 16 ? 4 : (P <= 32 ? 2 : 1); }
+
+// Bytes of dynamic LDS for mv_solve_fast: per pixel group, the active list,
+// NR rows at the fixed stride and the packed lower triangle.
 size_t mv_fast_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
-  const int na_max = subarray <= P ? n_ch : std::min(n_ch, 2 * P + 1);
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max +
-                                  static_cast<size_t>(P) * (P + 1) / 2);
+  return static_cast<size_t>(mv_fast_groups(P)) *
+         (ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * mv_fast_row_stride(n_ch, subarray, P) +
+                                   static_cast<size_t>(P) * (P + 1) / 2));
 }
 
 __global__ void mv_zero_counts(int* counts) {
@@ -709,49 +764,61 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
 // * No back substitution: with w = R^-1 1, sum w = 1^H R^-1 1 = |y1|^2 and
 //   w^H avg = y1^H y2 (y1 = L^-1 1, y2 = L^-1 avg), so
 //   out = (y1^H y2) / |y1|^2 * n.
-template <int P, int NR>
+template <int P, int NR, int RS, int G>
 __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
                                                     const uint32_t* __restrict__ mask,
                                                     float2* __restrict__ frame, float* acc,
                                                     const int* __restrict__ counts,
-                                                    const int* __restrict__ lists, int na_max) {
-  static_assert(P <= 64, "one matrix row per lane");
+                                                    const int* __restrict__ lists) {
+  constexpr int LP = 64 / G;  // lanes per pixel
+  static_assert(P <= LP, "one matrix row per lane of the pixel's group");
   constexpr int BUCKET = P / 16 - 1;
+  constexpr int TRI = P * (P + 1) / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x;
+  const int g = lane / LP, sub = lane % LP;
   const int n_ch = M.n_ch;
   const int n_pix = M.n0 * M.n1;
-  int* act = reinterpret_cast<int*>(smem_raw);
-  float2* rows = reinterpret_cast<float2*>(smem_raw + (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16);
-  // lower triangle, packed by rows: (i, k <= i) at i (i + 1) / 2 + k
-  float2* A = rows + static_cast<size_t>(NR) * na_max;
-  auto tri = [](int i) { return (i * (i + 1)) >> 1; };
+  // per group: act (n_ch ints), NR rows at stride RS, the lower triangle
+  // packed by columns -- (i >= k, k) at k P - k (k + 1) / 2 + i, so a
+  // column's rows are contiguous (lane-consecutive reads and stores)
+  const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
+  unsigned char* mine = smem_raw + g * (ints + sizeof(float2) * (NR * RS + TRI));
+  int* act = reinterpret_cast<int*>(mine);
+  float2* rows = reinterpret_cast<float2*>(mine + ints);
+  float2* A = rows + NR * RS;
+  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
 
   const int count = counts[BUCKET];
-  for (int item = blockIdx.x; item < count; item += gridDim.x) {
-    const int pix = lists[static_cast<size_t>(BUCKET) * n_pix + item];
-    __syncthreads();  // LDS of the previous pixel is free
-    const Staged sg = stage_pixel_nr<NR>(M, Y, mask, pix, lane, act, rows);
+  const int* list = lists + static_cast<size_t>(BUCKET) * n_pix;
+  for (int it = blockIdx.x * G; it < count; it += gridDim.x * G) {
+    // a group past the end of the list repeats the wavefront's first pixel
+    // and stores nothing
+    const bool valid = it + g < count;
+    const int pix = list[valid ? it + g : it];
+    __syncthreads();  // LDS of the previous pixels is free
+    const Staged sg = stage_group<NR, RS, G>(M, Y, mask, pix, lane, act, rows);
     const int na = sg.na, nrow = sg.nrow;
-    const float2* yc = Y + static_cast<size_t>(pix) * n_ch;
     const int L = min(max(M.subarray, 2), max(2, na / 2));
     const int ns = na - L + 1;
     const float inv = 1.0f / static_cast<float>(nrow * ns);
+    const int Lw = wave_max(L), nsw = wave_max(ns);  // uniform loop bounds
 
     // ---- covariance: first column, then the sub-diagonal walks ---------------
     // Loops have uniform bounds and fixed row counts (NR rows, the missing
     // ones zero) so the loads of a step are all issued before its products;
-    // lanes past their diagonal's end read clamped indices and store nothing.
-    double tr = 0.0;  // lane 0 walks the diagonal and sums the trace in FP64
+    // lanes past their diagonal's end (or their pixel's size) read clamped
+    // indices and store nothing.
+    double tr = 0.0;  // lane 0 of a group walks the diagonal, trace in FP64
     float cr, ci;
     {
       float ar[2] = {0.0f, 0.0f}, ai[2] = {0.0f, 0.0f};
-      const int m = min(lane, L - 1);
-      for (int a = 0; a < ns; a += 4) {  // R(m,0) += x[a+m] conj(x[a])
+      const int m = min(sub, L - 1);
+      for (int a = 0; a < nsw; a += 4) {  // R(m,0) += x[a+m] conj(x[a])
         float2 p[NR][4], q[NR][4];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          const float2* x = rows + r * na;
+          const float2* x = rows + r * RS;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int au = min(a + u, ns - 1);
@@ -772,20 +839,20 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       cr = ar[0] + ar[1];
       ci = ai[0] + ai[1];
     }
-    if (lane < L) A[tri(lane)] = make_float2(cr * inv, lane == 0 ? 0.0f : ci * inv);
-    if (lane == 0) tr = static_cast<double>(cr * inv);
+    if (sub < L) A[at(sub, 0)] = make_float2(cr * inv, sub == 0 ? 0.0f : ci * inv);
+    if (sub == 0) tr = static_cast<double>(cr * inv);
 #pragma unroll 2
-    for (int t = 0; t < L - 1; ++t) {
-      // R(lane+t+1, t+1) = R(lane+t, t) + sum_r (hi conj(hk) - lo conj(lk))
-      const int il = min(lane + t, L - 2);
+    for (int t = 0; t < Lw - 1; ++t) {
+      // R(sub+t+1, t+1) = R(sub+t, t) + sum_r (hi conj(hk) - lo conj(lk))
+      const int il = min(sub + t, L - 2), tk = min(t, L - 2);
       float2 hi[NR], hk[NR], lo[NR], lk[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const float2* x = rows + r * na;
+        const float2* x = rows + r * RS;
         hi[r] = x[il + ns];
-        hk[r] = x[t + ns];
+        hk[r] = x[tk + ns];
         lo[r] = x[il];
-        lk[r] = x[t];
+        lk[r] = x[tk];
       }
       float dr = 0.0f, di = 0.0f;
 #pragma unroll
@@ -798,18 +865,18 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       cr += dr;
       ci += di;
       const float vr = cr * inv;
-      if (lane + t + 1 < L) A[tri(lane + t + 1) + t + 1] = make_float2(vr, lane == 0 ? 0.0f : ci * inv);
-      if (lane == 0) tr += static_cast<double>(vr);
+      if (sub + t + 1 < L) A[at(sub + t + 1, t + 1)] = make_float2(vr, sub == 0 ? 0.0f : ci * inv);
+      if (sub == 0 && t + 1 < L) tr += static_cast<double>(vr);
     }
     // ---- diagonal loading max(delta tr / L, 1e-20) ---------------------------
-    tr = __shfl(tr, 0, 64);
+    tr = group_bcast_d<G>(tr, 0, lane);
     const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
     __syncthreads();
     // subarray average of the centre vector: the second right-hand side
     float b2r = 0.0f, b2i = 0.0f;
-    if (lane < L) {
-      A[tri(lane) + lane].x += eps;
-      const float2* cen = rows + sg.crow * na + lane;
+    if (sub < L) {
+      A[at(sub, sub)].x += eps;
+      const float2* cen = rows + sg.crow * RS + sub;
       float pr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       for (int s0 = 0; s0 < ns; s0 += 8) {
         float2 v[8];
@@ -837,102 +904,116 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     // by readlane from the registers of lane j0+c), so every sum runs over k
     // in increasing order as in the reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
-    float b1r = lane < L ? 1.0f : 0.0f, b1i = 0.0f;
-    bool ok = true;
-    for (int j0 = 0; j0 < L && ok; j0 += CB) {
+    float b1r = sub < L ? 1.0f : 0.0f, b1i = 0.0f;
+    bool ok = true;  // this group's pivots so far are positive
+    for (int j0 = 0; j0 < Lw; j0 += CB) {
+      // Lanes outside [j0, L) keep zero partial sums, so their factor entries
+      // are 0 and the substitution updates below leave their b unchanged.
       float sr[CB], si[CB], lr[CB], li[CB];
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         sr[c] = 0.0f;
         si[c] = 0.0f;
-        lr[c] = 0.0f;
-        li[c] = 0.0f;
       }
-      if (lane >= j0 && lane < L) {
-        const float2* ri_ = A + tri(lane);
+      if (sub >= j0 && sub < L) {
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-          if (j0 + c <= lane) {
-            sr[c] = ri_[j0 + c].x;
-            si[c] = ri_[j0 + c].y;
+          if (j0 + c <= sub) {
+            const float2 v = A[at(sub, j0 + c)];
+            sr[c] = v.x;
+            si[c] = v.y;
           }
         }
 #pragma unroll 2
         for (int k = 0; k < j0; ++k) {
-          const float2 a = ri_[k];
+          // column k: this lane's row, and the block's rows j0 .. j0+CB-1 (contiguous;
+          // rows >= L hold stale values whose sums are never used, and j0 + CB <= P)
+          const float2* colk = A + at(0, k);
+          const float2 a = colk[sub];
 #pragma unroll
           for (int c = 0; c < CB; ++c) {
-            const float2 b = A[tri(min(j0 + c, L - 1)) + k];  // -= a conj(b)
+            const float2 b = colk[j0 + c];  // -= a conj(b)
             sr[c] = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr[c]));
             si[c] = fmaf(-a.y, b.x, fmaf(a.x, b.y, si[c]));
           }
         }
       }
+      // Column by column, branch-free (selects): the pivot row takes L_jj and
+      // y_j, rows below take L_ij and b_i -= L_ij y_j, rows above keep b. A
+      // group whose pixel has no column j (j >= L) changes nothing.
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         const int j = j0 + c;
-        if (j >= L) break;
+        if (j >= Lw) break;
+        const bool col = j < L;
 #pragma unroll
         for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
-          const float br = lane_bcast(lr[q], j), bi = lane_bcast(li[q], j);
+          const float br = group_bcast<G>(lr[q], j, lane), bi = group_bcast<G>(li[q], j, lane);
           sr[c] = fmaf(-lr[q], br, fmaf(-li[q], bi, sr[c]));
           si[c] = fmaf(-li[q], br, fmaf(lr[q], bi, si[c]));
         }
-        const float d = lane_bcast(sr[c], j);
-        if (!(d > 0.0f)) {
-          ok = false;
-          break;
-        }
+        const float d = group_bcast<G>(sr[c], j, lane);
+        ok = ok && (!col || d > 0.0f);
         // 1 / L_jj in one instruction (v_rsq_f32, ~1 ulp); L_jj itself is
         // only stored, never read back.
         const float rj = __builtin_amdgcn_rsqf(d);
-        const float y1r = lane_bcast(b1r, j) * rj, y1i = lane_bcast(b1i, j) * rj;
-        const float y2r = lane_bcast(b2r, j) * rj, y2i = lane_bcast(b2i, j) * rj;
-        if (lane == j) {
-          lr[c] = d * rj;
-          li[c] = 0.0f;
-          b1r = y1r;
-          b1i = y1i;
-          b2r = y2r;
-          b2i = y2i;
-        } else if (lane > j && lane < L) {
-          lr[c] = sr[c] * rj;
-          li[c] = si[c] * rj;
-          b1r = fmaf(-lr[c], y1r, fmaf(li[c], y1i, b1r));  // b -= l y
-          b1i = fmaf(-lr[c], y1i, fmaf(-li[c], y1r, b1i));
-          b2r = fmaf(-lr[c], y2r, fmaf(li[c], y2i, b2r));
-          b2i = fmaf(-lr[c], y2i, fmaf(-li[c], y2r, b2i));
-        }
+        const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
+        const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
+        const bool piv = col && sub == j, below = col && sub > j;
+        const float ar = sr[c] * rj, ai = si[c] * rj;
+        lr[c] = piv ? d * rj : ar;
+        li[c] = piv ? 0.0f : ai;
+        const float n1r = fmaf(-ar, y1r, fmaf(ai, y1i, b1r));  // b -= l y
+        const float n1i = fmaf(-ar, y1i, fmaf(-ai, y1r, b1i));
+        const float n2r = fmaf(-ar, y2r, fmaf(ai, y2i, b2r));
+        const float n2i = fmaf(-ar, y2i, fmaf(-ai, y2r, b2i));
+        b1r = piv ? y1r : (below ? n1r : b1r);
+        b1i = piv ? y1i : (below ? n1i : b1i);
+        b2r = piv ? y2r : (below ? n2r : b2r);
+        b2i = piv ? y2i : (below ? n2i : b2i);
       }
-      if (lane >= j0 && lane < L) {
+      if (sub >= j0 && sub < L) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
-          if (j0 + c < L && j0 + c <= lane) A[tri(lane) + j0 + c] = make_float2(lr[c], li[c]);
+          if (j0 + c < L && j0 + c <= sub) A[at(sub, j0 + c)] = make_float2(lr[c], li[c]);
       }
       __syncthreads();
+      if (!__any(ok)) break;  // every group's factorisation failed
     }
-    float2 out;
-    bool das = !ok;
-    if (ok) {
-      const bool on = lane < L;
-      const float dn = wsum(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);       // |y1|^2 = sum w
-      const float nr = wsum(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);       // y1^H y2 = w^H avg
-      const float ni = wsum(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
-      if (dn < 1e-30f) {
-        das = true;
-      } else {
-        const float nf = static_cast<float>(na);
-        out = make_float2(nr / dn * nf, ni / dn * nf);
+    const bool on = sub < L;
+    const float dn = group_sum<G>(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);   // |y1|^2 = sum w
+    const float nr = group_sum<G>(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);   // y1^H y2 = w^H avg
+    const float ni = group_sum<G>(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
+    const bool das = !ok || !(dn >= 1e-30f);
+    const float nf = static_cast<float>(na);
+    float2 out = make_float2(nr / dn * nf, ni / dn * nf);
+    // DAS fallback and coherence factor over the group's active centre vector
+    // (the staged centre row: yc[act[a]] for a < na)
+    const float2* cen = rows + sg.crow * RS;
+    float sr = 0.0f, si = 0.0f, inc = 0.0f;
+    for (int a = sub; a < na; a += LP) {
+      const float2 v = cen[a];
+      sr += v.x;
+      si += v.y;
+      if (M.cf) {
+        const float m = hypotf(v.x, v.y);
+        inc += m * m;
       }
     }
-    if (das) out = das_sum(yc, act, na, lane);
-    if (M.status && lane == 0 && das) M.status[pix] = 2;
+    sr = group_sum<G>(sr);
+    si = group_sum<G>(si);
+    if (das) out = make_float2(sr, si);
     if (M.cf) {
-      const float f = coherence(yc, act, na, lane);
+      inc = group_sum<G>(inc);
+      const float mc = hypotf(sr, si);
+      const float f = (inc > 0.0f) ? (mc * mc) / (nf * inc) : 0.0f;
       out.x *= f;
       out.y *= f;
     }
-    if (lane == 0) store_out(M, pix, out, frame, acc);
+    if (valid && sub == 0) {
+      if (M.status && das) M.status[pix] = 2;
+      store_out(M, pix, out, frame, acc);
+    }
   }
 }
 
@@ -940,7 +1021,7 @@ template <int P>
 static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
                           float* acc, const int* counts, const int* lists, hipStream_t st) {
   const bool fast = P <= 64 && M.precision == 0 && !M.refine && M.weights == nullptr &&
-                    M.range_avg >= 0 && M.range_avg <= 2;
+                    M.range_avg >= 0 && M.range_avg <= 2 && mv_fast_row_stride(M.n_ch, M.subarray, P) > 0;
   const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
                            : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
@@ -951,7 +1032,7 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
   // and by the tuned occupancy when one is given.
   const int cap = M.occupancy > 0 ? M.occupancy : 32;
   const int lds_cap = static_cast<int>(160 * 1024 / smem);
-#define USX_LAUNCH(KERNEL)                                                                        \
+#define USX_LAUNCH(KERNEL, ...)                                                                   \
   do {                                                                                           \
     static const int vgpr_cap = [] {                                                             \
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(KERNEL),                           \
@@ -965,19 +1046,32 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
     const int per_cu = std::max(1, std::min(std::min(cap, lds_cap), vgpr_cap));                  \
     const int grid = std::min(n_pix, 256 * per_cu);                                              \
     hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(64), smem, st, M, Y, mask, frame, acc, counts,   \
-                       lists, na_max);                                                           \
+                       lists __VA_OPT__(, ) __VA_ARGS__);                                        \
   } while (0)
   if constexpr (P <= 64) {
     if (fast) {
-      if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1>));
-      else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3>));
-      else USX_LAUNCH((mv_solve_fast<P, 5>));
+      constexpr int G = P <= 16 ? 4 : (P <= 32 ? 2 : 1);  // mv_fast_groups(P)
+      constexpr int RT = mv_tight_stride(P) < 128 ? mv_tight_stride(P) : 128;
+      const int rs = mv_fast_row_stride(M.n_ch, M.subarray, P);
+      if (rs == RT && RT < 128) {
+        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, RT, G>));
+        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, RT, G>));
+        else USX_LAUNCH((mv_solve_fast<P, 5, RT, G>));
+      } else if (rs == 128) {
+        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, 128, G>));
+        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, 128, G>));
+        else USX_LAUNCH((mv_solve_fast<P, 5, 128, G>));
+      } else {
+        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, 256, G>));
+        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, 256, G>));
+        else USX_LAUNCH((mv_solve_fast<P, 5, 256, G>));
+      }
       return;
     }
   }
-  if (M.precision == 1) USX_LAUNCH((mv_solve<P, 1>));
-  else if (M.precision == 2) USX_LAUNCH((mv_solve<P, 2>));
-  else USX_LAUNCH((mv_solve<P, 0>));
+  if (M.precision == 1) USX_LAUNCH((mv_solve<P, 1>), na_max);
+  else if (M.precision == 2) USX_LAUNCH((mv_solve<P, 2>), na_max);
+  else USX_LAUNCH((mv_solve<P, 0>), na_max);
 #undef USX_LAUNCH
 }
 
