@@ -814,13 +814,13 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     {
       float ar[2] = {0.0f, 0.0f}, ai[2] = {0.0f, 0.0f};
       const int m = min(sub, L - 1);
-      for (int a = 0; a < nsw; a += 4) {  // R(m,0) += x[a+m] conj(x[a])
-        float2 p[NR][4], q[NR][4];
+      for (int a = 0; a < nsw; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
+        float2 p[NR][2], q[NR][2];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           const float2* x = rows + r * RS;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < 2; ++u) {
             const int au = min(a + u, ns - 1);
             p[r][u] = x[au + m];
             q[r][u] = a + u < ns ? x[au] : make_float2(0.0f, 0.0f);
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < 2; ++u) {
             const float2 pp = p[r][u], qq = q[r][u];
             ar[u & 1] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u & 1]));
             ai[u & 1] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u & 1]));
