@@ -45,8 +45,8 @@ constexpr int kKB = kF / 32;         // 8 K blocks
 constexpr int kLayerBf16 = kF * kF;  // swizzled bf16 per layer
 constexpr int kNT = 1;               // 16-sample tiles per wavefront (training)
 constexpr int kNTi = 1;              // 16-sample tiles per wavefront (inference)
-constexpr int kInferWaves = 4;       // wavefronts per inference workgroup
-constexpr int kStageTiles = 4;       // output tiles per LDS stage (of kMT per layer)
+constexpr int kInferWaves = 8;       // wavefronts per inference workgroup
+constexpr int kStageTiles = 8;       // output tiles per LDS stage (of kMT per layer)
 
 // FP32 -> BF16, round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint32_t f2bf(float x) {
@@ -56,22 +56,34 @@ __device__ __forceinline__ uint32_t f2bf(float x) {
   return u >> 16;
 }
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ uint32_t pack2(float a, float b) { return f2bf(a) | (f2bf(b) << 16); }
+// Two floats -> packed BF16 pair (low = a), round to nearest even: one
+// v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  typedef float fl2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(fl2{a, b}, bf2));
+}
 
 __device__ __forceinline__ bf8 as_bf8(u4 v) { return __builtin_bit_cast(bf8, v); }
 
 // GELU in its tanh form, 0.5 z (1 + tanh(k (z + 0.044715 z^3))), k = sqrt(2 / pi),
 // and its derivative; tanh(u) = 1 - 2 / (1 + e^{2u}) (saturates correctly at
-// both ends: e^{2u} = inf gives 1, e^{2u} = 0 gives -1).
+// both ends: e^{2u} = inf gives 1, e^{2u} = 0 gives -1). The forward value is
+// evaluated as the same function in sigmoid form, z / (1 + e^{-2a}) with
+// a = k (z + c z^3), i.e. z / (1 + 2^(z (p + q z^2))), p = -2 k log2(e),
+// q = p c: five VALU operations and two transcendentals (-inf..inf maps to
+// -0..inf as the tanh form does).
 constexpr float kGeluK = 0.7978845608028654f;
 constexpr float kGeluC = 0.044715f;
+constexpr float kGeluP = static_cast<float>(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+constexpr float kGeluQ = static_cast<float>(-2.0 * 0.7978845608028654 * 1.4426950408889634 * 0.044715);
 __device__ __forceinline__ float tanh_fast(float u) {
   const float e = __expf(2.0f * u);
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float gelu(float z) {
-  const float t = tanh_fast(kGeluK * (z + kGeluC * z * z * z));
-  return 0.5f * z * (1.0f + t);
+  const float u = z * __builtin_fmaf(kGeluQ, z * z, kGeluP);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
 }
 __device__ __forceinline__ float gelu_grad(float z) {
   const float t = tanh_fast(kGeluK * (z + kGeluC * z * z * z));
@@ -126,6 +138,47 @@ __device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kb = 0; kb < kKB; ++kb) cur[kb] = nxt[kb];
+  }
+}
+
+// Inference stage: output tiles M0 .. M0 + MT - 1 from the LDS copy w of
+// their fragments. The MT tiles advance together one K block at a time, so
+// the wavefront keeps MT independent accumulator chains in flight, and the
+// fragments of block kb + 1 are read while block kb's MFMAs run. Each tile
+// still accumulates K blocks 0..7 in order onto its bias.
+template <int M0, int MT, int NT>
+__device__ __forceinline__ void stage_tiles(const u4* w, const float* bias,
+                                            const u4 (&act)[kKB][NT], f4 (&acc)[kMT][NT],
+                                            int lane, uint32_t kmask) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < MT; ++i) {
+    const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * (M0 + i) + 4 * g);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[M0 + i][t] = f4{bb.x, bb.y, bb.z, bb.w};
+  }
+  u4 cur[MT], nxt[MT];
+  if (kmask & 1u) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i) cur[i] = w[(i * kKB) * 64 + lane];
+  }
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+    if (kb + 1 < kKB && ((kmask >> (kb + 1)) & 1u)) {
+#pragma unroll
+      for (int i = 0; i < MT; ++i) nxt[i] = w[(i * kKB + kb + 1) * 64 + lane];
+    }
+    if ((kmask >> kb) & 1u) {
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[M0 + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              as_bf8(cur[i]), as_bf8(act[kb][t]), acc[M0 + i][t], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < MT; ++i) cur[i] = nxt[i];
   }
 }
 
@@ -210,56 +263,82 @@ __device__ __forceinline__ void activate(const f4 (&acc)[kMT][NT], u4 (&act)[kKB
 }  // namespace
 
 // Weights of the inference kernel move HBM/L2 -> registers -> LDS one stage
-// (kStageTiles output tiles of a layer) at a time; each thread carries
-// kPre 16-byte rows of the next stage in registers while the current stage's
-// MFMAs run, so the global loads overlap the compute.
+// (kStageTiles output tiles of a layer) at a time, into two LDS buffers used
+// alternately: each thread fetches its kPre 16-byte rows of the next stage
+// into registers before the current stage's MFMAs and stores them into the
+// other buffer after them, so the global loads overlap the compute and one
+// barrier per stage suffices.
 constexpr int kStageU4 = kStageTiles * kKB * 64;
 constexpr int kPre = kStageU4 / (64 * kInferWaves);
 static_assert(kMT % kStageTiles == 0 && kStageU4 % (64 * kInferWaves) == 0, "stage shape");
+static_assert((kMT / kStageTiles) % 2 == 0, "stages per layer must alternate the buffers evenly");
 
 __device__ __forceinline__ void fetch_stage(const u4* wsrc, int l, int m0, int tid, u4 (&pre)[kPre]) {
   const u4* src = wsrc + l * (kLayerBf16 / 8) + m0 * kKB * 64;
 #pragma unroll
   for (int q = 0; q < kPre; ++q) pre[q] = src[tid + q * 64 * kInferWaves];
 }
+__device__ __forceinline__ void store_stage(u4* buf, int tid, const u4 (&pre)[kPre]) {
+#pragma unroll
+  for (int q = 0; q < kPre; ++q) buf[tid + q * 64 * kInferWaves] = pre[q];
+}
 
-// Layer l from output tile M0 on: store the prefetched stage, start fetching
-// the next one (this layer's next stage, else layer l + 1's first), run this
-// stage's tiles; compile-time recursion over the stages so the accumulator
-// tiles stay register-indexed.
+// Layer l from output tile M0 on: fetch the next stage (this layer's next, else
+// layer l + 1's first), run this stage's tiles from its buffer, store the next
+// stage into the other buffer (whose previous stage every wavefront finished
+// before the last barrier), barrier; compile-time recursion over the stages
+// so the accumulator tiles stay register-indexed.
 template <int M0>
 __device__ __forceinline__ void stage_and_run(u4* wl, const u4* wsrc, int l, const float* bias,
                                               const u4 (&act)[kKB][kNTi], f4 (&acc)[kMT][kNTi],
-                                              u4 (&pre)[kPre], int tid, int lane) {
+                                              int tid, int lane, uint32_t kmask) {
   if constexpr (M0 < kMT) {
-    __syncthreads();  // every wavefront is done reading the previous stage
-#pragma unroll
-    for (int q = 0; q < kPre; ++q) wl[tid + q * 64 * kInferWaves] = pre[q];
+    constexpr int B = (M0 / kStageTiles) & 1;
+    u4 pre[kPre];
+    const bool next_layer = M0 + kStageTiles >= kMT;
+    const bool more = !next_layer || l + 1 < kL;
+    if (more) fetch_stage(wsrc, next_layer ? l + 1 : l, next_layer ? 0 : M0 + kStageTiles, tid, pre);
+    stage_tiles<M0, kStageTiles, kNTi>(wl + B * kStageU4, bias, act, acc, lane, kmask);
+    if (more) store_stage(wl + (B ^ 1) * kStageU4, tid, pre);
     __syncthreads();
-    if (M0 + kStageTiles < kMT) fetch_stage(wsrc, l, M0 + kStageTiles, tid, pre);
-    else if (l + 1 < kL) fetch_stage(wsrc, l + 1, 0, tid, pre);
-    layer<false, kNTi, M0, M0 + kStageTiles>(wl, bias, act, acc, lane);
-    stage_and_run<M0 + kStageTiles>(wl, wsrc, l, bias, act, acc, pre, tid, lane);
+    stage_and_run<M0 + kStageTiles>(wl, wsrc, l, bias, act, acc, tid, lane, kmask);
   }
+}
+
+// Channels active at any pixel of the wavefront (wave-uniform).
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) v |= __shfl_xor(v, o, 64);
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // ---------------------------------------------------------------------------
 // Inference: one workgroup = kInferWaves wavefronts x 16 kNTi pixels. Each
 // layer's swizzled weights pass through LDS kStageTiles output tiles at a
-// time (32 KB for 4 of the 16 tiles), shared by the workgroup's wavefronts;
-// the next stage is fetched into registers while the current one computes.
+// time (double-buffered), shared by the workgroup's wavefronts. The pixels of
+// a wavefront are consecutive depths of one line, whose active apertures are
+// nested: the first layer skips the K blocks (16 channels) that none of them
+// has active, which adds exactly nothing.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
                                                          const float2* __restrict__ Y,
                                                          const uint32_t* __restrict__ mask,
                                                          float2* __restrict__ frame, float* acc_out,
                                                          int cf, int incoherent) {
-  extern __shared__ u4 wl[];  // kLayerBf16 bf16
+  extern __shared__ u4 wl[];  // 2 x kStageU4
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int g = lane >> 4, r = lane & 15;
   const long base = (static_cast<long>(blockIdx.x) * kInferWaves + wv) * (16 * kNTi);
+  const int nw = (n_ch + 31) / 32;
+  const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
+  {
+    u4 pre[kPre];
+    fetch_stage(wsrc, 0, 0, tid, pre);
+    store_stage(wl, tid, pre);
+  }
   u4 act[kKB][kNTi];
   int na[kNTi];
+  uint32_t un[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int t = 0; t < kNTi; ++t) {
     const long pix = base + 16 * t + r;
@@ -267,18 +346,27 @@ __global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N,
     na[t] = load_features(Y, mask, n_ch, pix, pix < n_pix, lane, x);
 #pragma unroll
     for (int kb = 0; kb < kKB; ++kb) act[kb][t] = x[kb];
+    if (pix < n_pix) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) un[q] |= mask[pix * nw + q];
+    }
   }
+  // bit kb: channels 16 kb .. 16 kb + 15 active at some pixel of the wavefront
+  uint32_t live = 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t u = wave_or(un[q]);
+    live |= ((u & 0xffffu) ? 1u : 0u) << (2 * q);
+    live |= ((u >> 16) ? 1u : 0u) << (2 * q + 1);
+  }
+  __syncthreads();
   f4 acc[kMT][kNTi];
-  const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
-  u4 pre[kPre];
-  fetch_stage(wsrc, 0, 0, tid, pre);
   for (int l = 0; l < kL; ++l) {
-    stage_and_run<0>(wl, wsrc, l, N.bias + l * kF, act, acc, pre, tid, lane);
+    stage_and_run<0>(wl, wsrc, l, N.bias + l * kF, act, acc, tid, lane, l == 0 ? live : 0xffu);
     if (l + 1 < kL) activate<kNTi>(acc, act);
   }
   // out = sum over active channels of (1 + o_c) y_c; fewer than 4 active
   // channels -> the DAS sum, as minimum variance does.
-  const int nw = (n_ch + 31) / 32;
 #pragma unroll
   for (int t = 0; t < kNTi; ++t) {
     const long pix = base + 16 * t + r;
@@ -670,7 +758,7 @@ __global__ __launch_bounds__(256) void apod_pack_kernel(ApodNet N) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-size_t apod_infer_smem() { return sizeof(u4) * kStageU4; }
+size_t apod_infer_smem() { return sizeof(u4) * 2 * kStageU4; }
 
 void launch_apod_pack(const ApodNet& N, hipStream_t st) {
   hipLaunchKernelGGL(apod_pack_kernel, dim3((kApodParams + 255) / 256), dim3(256), 0, st, N);
