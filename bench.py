@@ -13,8 +13,9 @@ The headline `value` is MV frames/s. DAS, DAS + coherence factor and DMAS
 frame rates on the same RF, the end-to-end (host RF in, host frame out)
 MV rate, the learned-apodization rate (trained network shipped in
 usx/data, with its gCNR on the cyst beside MV's and its BF16 MFMA roofline)
-and the uneven batched stream (64 frames of seed 1, DAS + MV each, one HIP
-graph) are reported under "extras".
+the uneven batched stream (64 frames of seed 1, DAS + MV each, one HIP
+graph) and colour flow / power Doppler / wall filter on a 16-frame ensemble
+of the benchmark grid are reported under "extras".
 
 roofline: the dominant kernel of the MV step (`mv`, covariance + solve) is
 priced against the FP32 MFMA peak with the reference's FLOP count
@@ -156,6 +157,64 @@ def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False, apod=No
     return wall * 1e3 / steps, ev_ms / steps, kern, out
 
 
+def doppler_leg(usx, torch, frame, grid, steps, warmup, n_frames=16, prf=5000.0):
+    """Colour flow, power Doppler and the wall filter on a 16-frame ensemble of
+    the benchmark grid, device-resident: the DAS frame under a slow-time phase
+    ramp (0.4 rad per frame) and a slight amplitude trend. Colour flow uses the
+    reference CLI's settings (polynomial wall filter of order 1, Loupas, 6 x 1
+    window). HBM roofline on compulsory bytes: the ensemble read once, the four
+    output maps written once (13 B per pixel)."""
+    n_pix = grid.n_axis0 * grid.n_axis1
+    base = np.asarray(frame, np.complex64)
+    ens = np.stack([base * np.complex64(np.exp(0.4j * e) * (1.0 + 0.01 * e))
+                    for e in range(n_frames)]).astype(np.complex64)
+    dc = usx.DopplerConfig()
+    dc.wall_filter = usx.WallFilter.POLYNOMIAL
+    dc.polynomial_order = 1
+    dc.kernel_axial = 6
+    dc.kernel_lateral = 1
+    runner = usx.DopplerRunner(grid, n_frames, prf, 1540.0, 5.0e6, dc)
+    d_ens = torch.from_numpy(ens).to("cuda:0")
+    shape = (grid.n_axis0, grid.n_axis1)
+    v, pw, w = (torch.empty(shape, dtype=torch.float32, device="cuda:0") for _ in range(3))
+    ok = torch.empty(shape, dtype=torch.uint8, device="cuda:0")
+    filt = torch.empty_like(d_ens)
+    calls = {
+        "color_flow": lambda: runner.color_flow(d_ens.data_ptr(), v.data_ptr(), pw.data_ptr(),
+                                                w.data_ptr(), ok.data_ptr()),
+        "power_doppler": lambda: runner.power_doppler(d_ens.data_ptr(), pw.data_ptr()),
+        "wall_filter": lambda: runner.wall_filter(d_ens.data_ptr(), filt.data_ptr()),
+    }
+    out = {"frames": n_frames, "grid": list(shape)}
+    torch.cuda.synchronize()
+    for name, fn in calls.items():
+        for _ in range(warmup):
+            fn()
+        runner.synchronize()
+        runner.profile(True)
+        runner.record(0)
+        for _ in range(steps):
+            fn()
+        runner.record(1)
+        runner.synchronize()
+        k = json.loads(runner.profile_report())
+        runner.profile(False)
+        ms = runner.elapsed(0, 1) / steps
+        kms = k[name][1] / k[name][0]
+        out[f"{name}_ms"] = ms
+        out[f"{name}_kernel_ms"] = kms
+    cf_bytes = n_frames * n_pix * 8 + n_pix * 13
+    out["color_flow_hbm"] = {"bound": "hbm", "bytes_per_call": cf_bytes,
+                             "achieved_GBps": cf_bytes / (out["color_flow_kernel_ms"] * 1e-3) / 1e9,
+                             "frac": cf_bytes / (out["color_flow_kernel_ms"] * 1e-3) / PEAK_HBM}
+    wf_bytes = 2 * n_frames * n_pix * 8
+    out["wall_filter_hbm_frac"] = wf_bytes / (out["wall_filter_kernel_ms"] * 1e-3) / PEAK_HBM
+    pd_bytes = n_frames * n_pix * 8 + n_pix * 4
+    out["power_doppler_hbm_frac"] = pd_bytes / (out["power_doppler_kernel_ms"] * 1e-3) / PEAK_HBM
+    out["valid_fraction"] = float(ok.float().mean().item())
+    return out
+
+
 def time_host(usx, raw, grid, pc, steps):
     """End-to-end: host ChannelData in, host Frame out (PCIe included)."""
     pipe = usx.ImagingPipeline(grid, pc)
@@ -289,9 +348,12 @@ def main():
                                                   a.warmup, profile=True)
     log(f"  {mv_ms:.3f} ms/frame (events {mv_ev_ms:.3f}), kernels {kern}")
     extras = {"mv_fps": 1e3 / mv_ms}
+    das_frame = None
     for name, comb, cf in [("das", "DAS", False), ("das_cf", "DAS", True), ("dmas", "DMAS", False)]:
-        ms, ev_ms, k, _ = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
-                                      max(a.steps, 100), a.warmup, profile=True)
+        ms, ev_ms, k, fr = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
+                                       max(a.steps, 100), a.warmup, profile=True)
+        if name == "das":
+            das_frame = fr
         extras[f"{name}_fps"] = 1e3 / ms
         extras[f"{name}_kernels_ms"] = {n: v[1] / v[0] for n, v in k.items()}
         log(f"  {name}: {ms:.4f} ms/frame, kernels {k}")
@@ -333,6 +395,10 @@ def main():
     das_bytes = raw.rf.nbytes + grid.n_pixels * 8
     das_total_ms = sum(extras["das_kernels_ms"].values())
     extras["das_hbm_frac"] = das_bytes / (das_total_ms * 1e-3) / PEAK_HBM
+
+    log("Doppler (16-frame ensemble, device-resident) ...")
+    extras["doppler"] = doppler_leg(usx, torch, das_frame, grid, max(a.steps, 50), a.warmup)
+    log(f"  {extras['doppler']}")
 
     if not a.skip_stream:
         extras["stream"] = run_stream(a.stream_frames, a.tile_cache)

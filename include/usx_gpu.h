@@ -18,6 +18,10 @@
  *                                core/include/usx/pipeline.hpp:52,
  *                                core/src/pipeline.cpp:25-46
  *
+ * and, downstream of the frames, the estimators of core/include/usx/doppler.hpp
+ * (usx_wall_filter, usx_color_flow, usx_power_doppler, usx_axial_displacement,
+ * usx_axial_strain; each cites its reference function).
+ *
  * Conventions
  *   - Plain pointers and sizes only; no C++ or torch types cross the ABI.
  *   - Channel data is [event][channel][sample], sample fastest
@@ -215,6 +219,69 @@ usx_status usx_display(usx_ctx* ctx, const usx_bf_desc* b, const usx_display_des
 usx_status usx_process_display(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
                                const usx_display_desc* p, const float* rf, float* image,
                                float* reference, int memory, usx_stage_times* times);
+
+/* ---- Doppler and speckle tracking (doppler.cpp) ------------------------ *
+ *
+ * On frames beamformed from IQ. An ensemble is n_frames frames of
+ * n_axis0 * n_axis1 complex pixels laid one after the other (frame-major,
+ * axis1 fastest within a frame). `reject` is the wall filter as the row-major
+ * n_frames x n_frames projector (I - P) that removes the least-squares fit of
+ * a low-order polynomial in slow time, built on the host exactly as
+ * polynomial_reject_matrix (doppler.cpp:13-46) builds it; NULL means no
+ * filter (WallFilter::None). */
+typedef struct {
+  int n_frames;            /* >= 1 */
+  int n_axis0, n_axis1;
+  const float* reject;     /* host, n_frames * n_frames, or NULL */
+} usx_ensemble_desc;
+
+/* apply_wall_filter (doppler.cpp:48-71): out = reject applied along slow
+ * time at every pixel. ensemble, out: n_frames * n_axis0 * n_axis1 complex. */
+usx_status usx_wall_filter(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c32* ensemble,
+                           usx_c32* out, int memory);
+
+/* power_doppler (doppler.cpp:178-186): power[k] = sum over the filtered
+ * frames of |z|^2. power: n_axis0 * n_axis1 floats. */
+usx_status usx_power_doppler(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c32* ensemble,
+                             float* power, int memory);
+
+/* DopplerConfig (doppler.hpp:40-79) and the frame facts color_flow reads. */
+typedef struct {
+  float prf, speed_of_sound;
+  float demodulation_frequency;  /* of the ensemble's first frame */
+  float dz;                      /* |axis1[1] - axis1[0]|, 1e-4 for a single depth */
+  int loupas;
+  int kernel_axial, kernel_lateral;   /* window half sizes; negative counts as 0 */
+  float coherence_threshold;
+  float power_factor;            /* 10^(power_threshold_db / 10), as std::pow(10.f, db / 10.f) */
+} usx_color_flow_desc;
+
+/* color_flow (doppler.cpp:73-176) after the wall filter of e: per pixel the
+ * axial velocity (m/s, + away from the probe), the post-filter power, the
+ * normalised spectral width and the valid flag (1: flow), each
+ * n_axis0 * n_axis1. Needs n_frames >= 2. */
+usx_status usx_color_flow(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_color_flow_desc* f,
+                          const usx_c32* ensemble, float* velocity, float* power, float* variance,
+                          uint8_t* valid, int memory);
+
+typedef struct {
+  int n_axis0, n_axis1;
+  float speed_of_sound;
+  float demodulation_frequency;  /* of the first frame */
+  float dz;                      /* as for color flow */
+  int kernel_axial, kernel_lateral, max_lag;  /* as passed; clamped as the reference does */
+} usx_displacement_desc;
+
+/* estimate_axial_displacement (doppler.cpp:188-269): axial displacement (m)
+ * and |normalised correlation| per pixel from two frames of one grid. */
+usx_status usx_axial_displacement(usx_ctx* ctx, const usx_displacement_desc* d,
+                                  const usx_c32* before, const usx_c32* after, float* axial,
+                                  float* correlation, int memory);
+
+/* axial_strain (doppler.cpp:271-303): least-squares slope of `axial` against
+ * depth over fit_length / 2 (at least 1) depths either side. */
+usx_status usx_axial_strain(usx_ctx* ctx, int n_axis0, int n_axis1, float dz, int fit_length,
+                            const float* axial, float* strain, int memory);
 
 /* ---- minimum-variance channel weights (no reference counterpart) ------- */
 

@@ -204,6 +204,35 @@ class SimCfg:
         return OcSimCfg(*[getattr(self, n) for n, _ in OcSimCfg._fields_])
 
 
+class OcDoppler(C.Structure):
+    _fields_ = [("wall_filter", _i), ("polynomial_order", _i), ("loupas", _i),
+                ("kernel_axial", _i), ("kernel_lateral", _i), ("power_threshold_db", _f),
+                ("coherence_threshold", _f)]
+
+
+@dataclass
+class Doppler:
+    """DopplerConfig (reference doppler.hpp:40-79); wall_filter 0 none,
+    1 polynomial regression, 2 mean subtraction."""
+    wall_filter: int = 1
+    polynomial_order: int = 1
+    loupas: int = 1
+    kernel_axial: int = 4
+    kernel_lateral: int = 1
+    power_threshold_db: float = -35.0
+    coherence_threshold: float = 0.35
+
+    @staticmethod
+    def of(d) -> "Doppler":
+        def e(v):
+            return int(v.value) if hasattr(v, "value") else int(v)
+        return Doppler(e(d.wall_filter), d.polynomial_order, int(bool(d.loupas)), d.kernel_axial,
+                       d.kernel_lateral, d.power_threshold_db, d.coherence_threshold)
+
+    def c(self) -> OcDoppler:
+        return OcDoppler(*[getattr(self, n) for n, _ in OcDoppler._fields_])
+
+
 class OracleError(RuntimeError):
     pass
 
@@ -397,6 +426,57 @@ class Oracle:
                                               stats))
         return frame, {"demodulate_ms": stats[0], "beamform_ms": stats[1],
                        "total_ms": stats[2]}
+
+    # ---- Doppler and speckle tracking (doppler.cpp) ------------------------
+
+    @staticmethod
+    def _ensemble(grid: GridSpec, frames):
+        return np.ascontiguousarray(np.asarray(frames, np.complex64).reshape(-1, grid.n0, grid.n1))
+
+    def wall_filter(self, grid: GridSpec, frames, wall: int, order: int = 1):
+        ens = self._ensemble(grid, frames)
+        out = np.zeros_like(ens)
+        self._check(self.lib.oc_wall_filter(C.byref(grid.c()), ens.shape[0], ens.ctypes.data_as(_fp),
+                                            wall, order, out.ctypes.data_as(_fp)))
+        return out
+
+    def power_doppler(self, grid: GridSpec, frames, wall: int = 1, order: int = 1):
+        ens = self._ensemble(grid, frames)
+        out = np.zeros((grid.n0, grid.n1), np.float32)
+        self._check(self.lib.oc_power_doppler(C.byref(grid.c()), ens.shape[0],
+                                              ens.ctypes.data_as(_fp), wall, order, _ptr(out)))
+        return out
+
+    def color_flow(self, grid: GridSpec, frames, fdem: float, prf: float, c: float, d: Doppler):
+        """Returns (velocity, power, variance, valid, nyquist_velocity)."""
+        ens = self._ensemble(grid, frames)
+        shape = (grid.n0, grid.n1)
+        v, p, w = (np.zeros(shape, np.float32) for _ in range(3))
+        ok = np.zeros(shape, np.uint8)
+        nyq = C.c_float(0.0)
+        self._check(self.lib.oc_color_flow(C.byref(grid.c()), ens.shape[0], ens.ctypes.data_as(_fp),
+                                           _f(fdem), _f(prf), _f(c), C.byref(d.c()), _ptr(v),
+                                           _ptr(p), _ptr(w), ok.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                           C.byref(nyq)))
+        return v, p, w, ok, nyq.value
+
+    def axial_displacement(self, grid: GridSpec, before, after, fdem: float, c: float,
+                           kernel_axial: int = 8, kernel_lateral: int = 2, max_lag: int = 8):
+        a = self._ensemble(grid, before)
+        b = self._ensemble(grid, after)
+        ax = np.zeros((grid.n0, grid.n1), np.float32)
+        co = np.zeros_like(ax)
+        self._check(self.lib.oc_axial_displacement(C.byref(grid.c()), a.ctypes.data_as(_fp),
+                                                   b.ctypes.data_as(_fp), _f(fdem), _f(c),
+                                                   kernel_axial, kernel_lateral, max_lag,
+                                                   _ptr(ax), _ptr(co)))
+        return ax, co
+
+    def axial_strain(self, grid: GridSpec, axial, fit_length: int = 15):
+        ax = np.ascontiguousarray(np.asarray(axial, np.float32).reshape(grid.n0, grid.n1))
+        out = np.zeros_like(ax)
+        self._check(self.lib.oc_axial_strain(C.byref(grid.c()), _ptr(ax), fit_length, _ptr(out)))
+        return out
 
 
 _cache: dict = {}

@@ -14,6 +14,7 @@
 
 #include "usx/beamformer.hpp"
 #include "usx/demodulate.hpp"
+#include "usx/doppler.hpp"
 #include "usx/geometry.hpp"
 #include "usx/pipeline.hpp"
 #include "usx/simulator.hpp"
@@ -126,6 +127,25 @@ usx::ChannelData make_data(const oc_setup* s, const oc_transmit* txs, int n_tx, 
   else
     std::memcpy(cd.rf.data(), data, n * sizeof(float));
   return cd;
+}
+
+// Frames of grid g from interleaved complex data (n_frames after each other).
+std::vector<usx::Frame> to_frames(const oc_grid* g, int n_frames, const float* data, float fdem) {
+  const usx::ScanGrid grid = to_grid(g);
+  const std::size_t n_pix = static_cast<std::size_t>(grid.n_pixels());
+  std::vector<usx::Frame> out(static_cast<std::size_t>(n_frames));
+  for (int f = 0; f < n_frames; ++f) {
+    usx::Frame& fr = out[static_cast<std::size_t>(f)];
+    fr.grid = grid;
+    fr.demodulation_frequency = fdem;
+    fr.allocate();
+    std::memcpy(static_cast<void*>(fr.data.data()), data + 2 * n_pix * static_cast<std::size_t>(f), n_pix * 8);
+  }
+  return out;
+}
+usx::WallFilter to_wall(int t) {
+  return t == 1 ? usx::WallFilter::PolynomialRegression
+                : (t == 2 ? usx::WallFilter::MeanSubtraction : usx::WallFilter::None);
 }
 
 template <typename F>
@@ -313,6 +333,72 @@ int oc_display(const oc_setup* s, const oc_grid* g, const oc_display_cfg* p, int
       const std::vector<usx::real> img = pipe.to_display(fr, med);
       std::memcpy(images_out + n * f, img.data(), n * sizeof(float));
     }
+  });
+}
+
+int oc_wall_filter(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
+                   float* out) {
+  return guarded([&] {
+    const std::vector<usx::Frame> f =
+        usx::apply_wall_filter(to_frames(g, n_frames, ensemble, 0.0f), to_wall(type), order);
+    for (std::size_t i = 0; i < f.size(); ++i)
+      std::memcpy(out + 2 * f[i].data.size() * i, f[i].data.data(), f[i].data.size() * 8);
+  });
+}
+
+int oc_power_doppler(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
+                     float* power_out) {
+  return guarded([&] {
+    const std::vector<float> p =
+        usx::power_doppler(to_frames(g, n_frames, ensemble, 0.0f), to_wall(type), order);
+    std::memcpy(power_out, p.data(), p.size() * sizeof(float));
+  });
+}
+
+int oc_color_flow(const oc_grid* g, int n_frames, const float* ensemble, float fdem, float prf,
+                  float c, const oc_doppler* d, float* velocity, float* power, float* variance,
+                  uint8_t* valid, float* nyquist) {
+  return guarded([&] {
+    usx::DopplerConfig dc;
+    dc.wall_filter = to_wall(d->wall_filter);
+    dc.polynomial_order = d->polynomial_order;
+    dc.loupas = d->loupas != 0;
+    dc.kernel_axial = d->kernel_axial;
+    dc.kernel_lateral = d->kernel_lateral;
+    dc.power_threshold_db = d->power_threshold_db;
+    dc.coherence_threshold = d->coherence_threshold;
+    const usx::ColorFlowMap m = usx::color_flow(to_frames(g, n_frames, ensemble, fdem), prf, c, dc);
+    const std::size_t n = m.velocity.size();
+    std::memcpy(velocity, m.velocity.data(), n * sizeof(float));
+    std::memcpy(power, m.power.data(), n * sizeof(float));
+    std::memcpy(variance, m.variance.data(), n * sizeof(float));
+    std::memcpy(valid, m.valid.data(), n);
+    *nyquist = m.nyquist_velocity;
+  });
+}
+
+int oc_axial_displacement(const oc_grid* g, const float* before, const float* after, float fdem,
+                          float c, int kernel_axial, int kernel_lateral, int max_lag, float* axial,
+                          float* correlation) {
+  return guarded([&] {
+    const std::vector<usx::Frame> a = to_frames(g, 1, before, fdem);
+    const std::vector<usx::Frame> b = to_frames(g, 1, after, fdem);
+    const usx::DisplacementMap m =
+        usx::estimate_axial_displacement(a[0], b[0], c, kernel_axial, kernel_lateral, max_lag);
+    std::memcpy(axial, m.axial.data(), m.axial.size() * sizeof(float));
+    std::memcpy(correlation, m.correlation.data(), m.correlation.size() * sizeof(float));
+  });
+}
+
+int oc_axial_strain(const oc_grid* g, const float* axial, int fit_length, float* strain) {
+  return guarded([&] {
+    usx::DisplacementMap m;
+    m.grid = to_grid(g);
+    const std::size_t n = static_cast<std::size_t>(m.grid.n_pixels());
+    m.axial.assign(axial, axial + n);
+    m.correlation.assign(n, 0.0f);
+    const std::vector<float> s = usx::axial_strain(m, fit_length);
+    std::memcpy(strain, s.data(), n * sizeof(float));
   });
 }
 

@@ -15,6 +15,8 @@
 //   beamform (geometry, delay-interpolate-rotate-accumulate, combiners,
 //   coherence factor, compounding)  core/src/beamformer.cpp:15-214, 297-549
 //   process_frame                   core/src/pipeline.cpp:25-46
+//   wall filter, colour flow, power Doppler, axial displacement and strain
+//                                   core/src/doppler.cpp:13-303
 //
 // Two numeric modes:
 //   FP32 exact order  -- every float expression is evaluated in the
@@ -728,6 +730,58 @@ std::vector<float> display_envelope(const Grid& G, const cf32* frame, float fdem
   return env;
 }
 
+// ---- Doppler and speckle tracking (doppler.cpp:13-303) ----------------------
+
+// The (I - P) slow-time projector of the polynomial wall filter
+// (doppler.cpp:13-46): orthonormalised monomials, subtracted outer products.
+std::vector<float> reject_matrix(int n, int order) {
+  order = std::clamp(order, 0, n - 1);
+  std::vector<std::vector<float>> basis;
+  for (int d = 0; d <= order; ++d) {
+    std::vector<float> v(static_cast<std::size_t>(n));
+    for (int k = 0; k < n; ++k) {
+      const float x = (n == 1) ? 0.0f : 2.0f * k / (n - 1) - 1.0f;
+      v[static_cast<std::size_t>(k)] = std::pow(x, d);
+    }
+    for (const auto& b : basis) {
+      float dot = 0;
+      for (int k = 0; k < n; ++k) dot += v[static_cast<std::size_t>(k)] * b[static_cast<std::size_t>(k)];
+      for (int k = 0; k < n; ++k) v[static_cast<std::size_t>(k)] -= dot * b[static_cast<std::size_t>(k)];
+    }
+    float nrm = 0;
+    for (float x : v) nrm += x * x;
+    nrm = std::sqrt(nrm);
+    if (nrm < 1e-8f) continue;
+    for (float& x : v) x /= nrm;
+    basis.push_back(std::move(v));
+  }
+  std::vector<float> M(static_cast<std::size_t>(n) * n, 0.0f);
+  for (int i = 0; i < n; ++i) M[static_cast<std::size_t>(i) * n + i] = 1.0f;
+  for (const auto& b : basis)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        M[static_cast<std::size_t>(i) * n + j] -= b[static_cast<std::size_t>(i)] * b[static_cast<std::size_t>(j)];
+  return M;
+}
+
+// Filtered ensemble (doppler.cpp:48-71); type 0 copies.
+std::vector<cf32> wall_filtered(const float* ens, int n, std::size_t n_pix, int type, int order) {
+  const cf32* z = reinterpret_cast<const cf32*>(ens);
+  std::vector<cf32> out(z, z + n_pix * static_cast<std::size_t>(n));
+  if (type == 0 || n == 0) return out;
+  const std::vector<float> M = reject_matrix(n, type == 2 ? 0 : order);
+  for (std::size_t k = 0; k < n_pix; ++k)
+    for (int i = 0; i < n; ++i) {
+      cf32 acc(0, 0);
+      for (int j = 0; j < n; ++j)
+        acc += z[static_cast<std::size_t>(j) * n_pix + k] * M[static_cast<std::size_t>(i) * n + j];
+      out[static_cast<std::size_t>(i) * n_pix + k] = acc;
+    }
+  return out;
+}
+
+float axial_step(const Grid& G) { return G.n1 > 1 ? std::abs(G.axis1[1] - G.axis1[0]) : 1e-4f; }
+
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -906,6 +960,182 @@ int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const 
       stats_out[1] = bm;
       stats_out[2] = std::chrono::duration<double, std::milli>(clk::now() - t_start).count();
     }
+  });
+}
+
+int oc_wall_filter(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
+                   float* out) {
+  return guarded([&] {
+    const Grid G = build_grid(*g);
+    const std::size_t n_pix = static_cast<std::size_t>(G.n0) * G.n1;
+    const std::vector<cf32> f = wall_filtered(ensemble, n_frames, n_pix, type, order);
+    std::memcpy(out, f.data(), f.size() * sizeof(cf32));
+  });
+}
+
+int oc_power_doppler(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
+                     float* power_out) {
+  return guarded([&] {
+    const Grid G = build_grid(*g);
+    const std::size_t n_pix = static_cast<std::size_t>(G.n0) * G.n1;
+    const std::vector<cf32> f = wall_filtered(ensemble, n_frames, n_pix, type, order);
+    for (std::size_t k = 0; k < n_pix; ++k) power_out[k] = 0.0f;
+    for (int e = 0; e < n_frames; ++e)
+      for (std::size_t k = 0; k < n_pix; ++k) power_out[k] += std::norm(f[static_cast<std::size_t>(e) * n_pix + k]);
+  });
+}
+
+// color_flow (doppler.cpp:73-176): per pixel the window (lateral outer, axial
+// inner) and, inside, the ensemble, in one running sum each.
+int oc_color_flow(const oc_grid* g, int n_frames, const float* ensemble, float fdem, float prf,
+                  float c, const oc_doppler* d, float* velocity, float* power, float* variance,
+                  uint8_t* valid, float* nyquist) {
+  return guarded([&] {
+    if (n_frames < 2) throw Invalid("color_flow: need at least 2 frames in the ensemble");
+    const Grid G = build_grid(*g);
+    const int n0 = G.n0, n1 = G.n1;
+    const std::size_t n_pix = static_cast<std::size_t>(n0) * n1;
+    const std::vector<cf32> f =
+        wall_filtered(ensemble, n_frames, n_pix, d->wall_filter, d->polynomial_order);
+    const float dz = axial_step(G);
+    *nyquist = (fdem > 0.0f) ? c * prf / (4.0f * fdem) : 0.0f;
+    const int ka = std::max(0, d->kernel_axial), kl = std::max(0, d->kernel_lateral);
+    auto at = [&](int e, std::size_t kk) { return f[static_cast<std::size_t>(e) * n_pix + kk]; };
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j) {
+        const std::size_t k = static_cast<std::size_t>(i) * n1 + j;
+        velocity[k] = power[k] = variance[k] = 0.0f;
+        valid[k] = 0;
+        cf32 r01(0, 0), r10(0, 0);
+        float r00 = 0.0f;
+        for (int ii = i - kl; ii <= i + kl; ++ii) {
+          if (ii < 0 || ii >= n0) continue;
+          for (int jj = j - ka; jj <= j + ka; ++jj) {
+            if (jj < 0 || jj >= n1) continue;
+            const std::size_t kk = static_cast<std::size_t>(ii) * n1 + jj;
+            for (int e = 0; e < n_frames; ++e) {
+              const cf32 z = at(e, kk);
+              r00 += std::norm(z);
+              if (e + 1 < n_frames) r01 += std::conj(z) * at(e + 1, kk);
+              if (jj + 1 < n1) r10 += std::conj(z) * at(e, kk + 1);
+            }
+          }
+        }
+        power[k] = r00;
+        if (r00 <= 0.0f) continue;
+        const float coh = std::abs(r01) / r00;
+        variance[k] = std::clamp(2.0f * (1.0f - coh), 0.0f, 1.0f);
+        if (coh < d->coherence_threshold) continue;
+        float fc = fdem;
+        if (d->loupas) {
+          const float fe = std::arg(r10) * c / (4.0f * kPiF * dz);
+          if (fe > 0.25f * fdem && fe < 2.0f * fdem) fc = fe;
+        }
+        if (fc <= 0.0f) continue;
+        velocity[k] = -std::arg(r01) * c * prf / (4.0f * kPiF * fc);
+        valid[k] = 1;
+      }
+    float pmax = 0.0f;
+    for (std::size_t k = 0; k < n_pix; ++k) pmax = std::max(pmax, power[k]);
+    if (pmax > 0.0f) {
+      const float thr = pmax * std::pow(10.0f, d->power_threshold_db / 10.0f);
+      for (std::size_t k = 0; k < n_pix; ++k)
+        if (power[k] < thr) valid[k] = 0;
+    }
+  });
+}
+
+// estimate_axial_displacement (doppler.cpp:188-269).
+int oc_axial_displacement(const oc_grid* g, const float* before, const float* after, float fdem,
+                          float c, int kernel_axial, int kernel_lateral, int max_lag, float* axial,
+                          float* correlation) {
+  return guarded([&] {
+    const Grid G = build_grid(*g);
+    const int n0 = G.n0, n1 = G.n1;
+    const cf32* a = reinterpret_cast<const cf32*>(before);
+    const cf32* b = reinterpret_cast<const cf32*>(after);
+    const float dz = axial_step(G);
+    const int ka = std::max(1, kernel_axial), kl = std::max(0, kernel_lateral);
+    const int lmax = std::max(0, max_lag);
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j) {
+        const std::size_t k = static_cast<std::size_t>(i) * n1 + j;
+        axial[k] = correlation[k] = 0.0f;
+        cf32 r10(0, 0);
+        for (int ii = i - kl; ii <= i + kl; ++ii) {
+          if (ii < 0 || ii >= n0) continue;
+          for (int jj = j - ka; jj <= j + ka; ++jj) {
+            if (jj < 0 || jj + 1 >= n1) continue;
+            const std::size_t kk = static_cast<std::size_t>(ii) * n1 + jj;
+            r10 += std::conj(a[kk]) * a[kk + 1];
+          }
+        }
+        float fc = fdem;
+        const float fe = std::arg(r10) * c / (4.0f * kPiF * dz);
+        if (fe > 0.25f * fdem && fe < 2.0f * fdem) fc = fe;
+        if (fc <= 0.0f) continue;
+        cf32 best(0, 0);
+        float best_corr = -1.0f;
+        int best_lag = 0;
+        for (int lag = -lmax; lag <= lmax; ++lag) {
+          cf32 r(0, 0);
+          float ea = 0.0f, eb = 0.0f;
+          for (int ii = i - kl; ii <= i + kl; ++ii) {
+            if (ii < 0 || ii >= n0) continue;
+            for (int jj = j - ka; jj <= j + ka; ++jj) {
+              const int jl = jj + lag;
+              if (jj < 0 || jj >= n1 || jl < 0 || jl >= n1) continue;
+              const cf32 x = a[static_cast<std::size_t>(ii) * n1 + jj];
+              const cf32 y = b[static_cast<std::size_t>(ii) * n1 + jl];
+              r += std::conj(x) * y;
+              ea += std::norm(x);
+              eb += std::norm(y);
+            }
+          }
+          if (ea <= 0.0f || eb <= 0.0f) continue;
+          const float corr = std::abs(r) / std::sqrt(ea * eb);
+          if (corr > best_corr) {
+            best_corr = corr;
+            best = r;
+            best_lag = lag;
+          }
+        }
+        if (best_corr < 0.0f) continue;
+        correlation[k] = std::clamp(best_corr, 0.0f, 1.0f);
+        axial[k] = static_cast<float>(best_lag) * dz + (-std::arg(best) * c / (4.0f * kPiF * fc));
+      }
+  });
+}
+
+// axial_strain (doppler.cpp:271-303): least-squares slope over the window.
+int oc_axial_strain(const oc_grid* g, const float* axial, int fit_length, float* strain) {
+  return guarded([&] {
+    const Grid G = build_grid(*g);
+    const int n0 = G.n0, n1 = G.n1;
+    const float dz = axial_step(G);
+    const int half = std::max(1, fit_length / 2);
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j) {
+        float sx = 0, sy = 0, sxx = 0, sxy = 0;
+        int n = 0;
+        for (int dd = -half; dd <= half; ++dd) {
+          const int jj = j + dd;
+          if (jj < 0 || jj >= n1) continue;
+          const float x = static_cast<float>(dd) * dz;
+          const float y = axial[static_cast<std::size_t>(i) * n1 + jj];
+          sx += x;
+          sy += y;
+          sxx += x * x;
+          sxy += x * y;
+          ++n;
+        }
+        float s = 0.0f;
+        if (n >= 2) {
+          const float den = n * sxx - sx * sx;
+          if (!(std::abs(den) < 1e-20f)) s = (n * sxy - sx * sy) / den;
+        }
+        strain[static_cast<std::size_t>(i) * n1 + j] = s;
+      }
   });
 }
 

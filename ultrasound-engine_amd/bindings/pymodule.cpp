@@ -5,12 +5,14 @@
 // simulator that feeds it, and the display helpers, so Python code written
 // against the reference runs unchanged. Array accessors are zero-copy views
 // tied to their owner (reference :32-42); work-heavy calls release the GIL.
-// Doppler, elastography and scan conversion are not part of this module.
+// Doppler and elastography follow reference :486-545; scan conversion is not
+// part of this module.
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -20,6 +22,7 @@
 #include "../core/src/gpu.hpp"
 #include "usx/beamformer.hpp"
 #include "usx/demodulate.hpp"
+#include "usx/doppler.hpp"
 #include "usx/geometry.hpp"
 #include "usx/pipeline.hpp"
 #include "usx/postprocess.hpp"
@@ -482,6 +485,89 @@ class FrameSequence {
 
 }  // namespace
 
+// Device-resident Doppler for benchmarks and device callers: a frame-major
+// ensemble already in device memory (e.g. a torch tensor's data_ptr()),
+// outputs to device pointers, enqueued on the runner's own stream.
+class DopplerRunner {
+ public:
+  DopplerRunner(const ScanGrid& grid, int n_frames, real prf, real speed_of_sound, real fdem,
+                const DopplerConfig& cfg)
+      : grid_(grid), n_frames_(n_frames) {
+    grid_.validate();
+    if (n_frames < 2) throw std::invalid_argument("color_flow: need at least 2 frames in the ensemble");
+    if (cfg.wall_filter != WallFilter::None)
+      reject_ = polynomial_reject_matrix(
+          n_frames, cfg.wall_filter == WallFilter::MeanSubtraction ? 0 : cfg.polynomial_order);
+    e_.n_frames = n_frames;
+    e_.n_axis0 = grid_.n_axis0;
+    e_.n_axis1 = grid_.n_axis1;
+    e_.reject = reject_.empty() ? nullptr : reject_.data();
+    f_.prf = prf;
+    f_.speed_of_sound = speed_of_sound;
+    f_.demodulation_frequency = fdem;
+    f_.dz = grid_.n_axis1 > 1 ? std::abs(grid_.axis1[1] - grid_.axis1[0]) : 1e-4f;
+    f_.loupas = cfg.loupas ? 1 : 0;
+    f_.kernel_axial = cfg.kernel_axial;
+    f_.kernel_lateral = cfg.kernel_lateral;
+    f_.coherence_threshold = cfg.coherence_threshold;
+    f_.power_factor = std::pow(10.0f, cfg.power_threshold_db / 10.0f);
+  }
+  void color_flow(std::uintptr_t ens, std::uintptr_t vel, std::uintptr_t pw, std::uintptr_t var,
+                  std::uintptr_t valid) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_color_flow(c, &e_, &f_, reinterpret_cast<const usx_c32*>(ens),
+                            reinterpret_cast<float*>(vel), reinterpret_cast<float*>(pw),
+                            reinterpret_cast<float*>(var), reinterpret_cast<std::uint8_t*>(valid),
+                            USX_DEVICE),
+          "color_flow");
+  }
+  void power_doppler(std::uintptr_t ens, std::uintptr_t pw) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_power_doppler(c, &e_, reinterpret_cast<const usx_c32*>(ens),
+                               reinterpret_cast<float*>(pw), USX_DEVICE),
+          "power_doppler");
+  }
+  void wall_filter(std::uintptr_t ens, std::uintptr_t out) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_wall_filter(c, &e_, reinterpret_cast<const usx_c32*>(ens),
+                             reinterpret_cast<usx_c32*>(out), USX_DEVICE),
+          "apply_wall_filter");
+  }
+  void synchronize() {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_ctx_synchronize(c), "synchronize");
+  }
+  void record(int slot) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_ctx_event_record(c, slot), "event_record");
+  }
+  float elapsed(int a, int b) {
+    usx_ctx* c = ctx_.get();
+    float ms = 0;
+    check(c, usx_ctx_event_elapsed(c, a, b, &ms), "event_elapsed");
+    return ms;
+  }
+  void profile(bool on) {
+    usx_ctx* c = ctx_.get();
+    check(c, usx_ctx_profile(c, on ? 1 : 0), "profile");
+  }
+  std::string profile_report() {
+    usx_ctx* c = ctx_.get();
+    std::vector<char> buf(1 << 16);
+    check(c, usx_ctx_profile_report(c, buf.data(), buf.size()), "profile_report");
+    return std::string(buf.data());
+  }
+  int n_frames() const { return n_frames_; }
+
+ private:
+  ScanGrid grid_;
+  int n_frames_;
+  std::vector<real> reject_;
+  usx_ensemble_desc e_{};
+  usx_color_flow_desc f_{};
+  GpuContext ctx_;
+};
+
 PYBIND11_MODULE(_usx, m) {
   m.doc() = "usx -- ultrasound imaging path on MI355X (gfx950 HIP kernels behind a C ABI)";
 
@@ -511,6 +597,10 @@ PYBIND11_MODULE(_usx, m) {
       .value("FP32", MvPrecision::FP32)
       .value("BF16", MvPrecision::BF16)
       .value("FP16", MvPrecision::FP16);
+  py::enum_<WallFilter>(m, "WallFilter")
+      .value("NONE", WallFilter::None)
+      .value("POLYNOMIAL", WallFilter::PolynomialRegression)
+      .value("MEAN", WallFilter::MeanSubtraction);
 
   py::class_<Vec3>(m, "Vec3")
       .def(py::init<>())
@@ -1016,6 +1106,107 @@ PYBIND11_MODULE(_usx, m) {
       .def("record", &FrameSequence::record, py::arg("slot"))
       .def("elapsed", &FrameSequence::elapsed, py::arg("a"), py::arg("b"))
       .def("__len__", &FrameSequence::size);
+  // --- doppler / elastography (reference :486-545) -------------------------
+  py::class_<DopplerConfig>(m, "DopplerConfig")
+      .def(py::init<>())
+      .def_readwrite("wall_filter", &DopplerConfig::wall_filter)
+      .def_readwrite("polynomial_order", &DopplerConfig::polynomial_order)
+      .def_readwrite("loupas", &DopplerConfig::loupas)
+      .def_readwrite("kernel_axial", &DopplerConfig::kernel_axial)
+      .def_readwrite("kernel_lateral", &DopplerConfig::kernel_lateral)
+      .def_readwrite("power_threshold_db", &DopplerConfig::power_threshold_db)
+      .def_readwrite("coherence_threshold", &DopplerConfig::coherence_threshold);
+
+  py::class_<ColorFlowMap>(m, "ColorFlowMap")
+      .def_readonly("grid", &ColorFlowMap::grid)
+      .def_readonly("nyquist_velocity", &ColorFlowMap::nyquist_velocity)
+      .def_property_readonly("velocity",
+                             [](py::object self) {
+                               ColorFlowMap& c = self.cast<ColorFlowMap&>();
+                               return view<real>(c.velocity.data(), {c.grid.n_axis0, c.grid.n_axis1}, self);
+                             })
+      .def_property_readonly("power",
+                             [](py::object self) {
+                               ColorFlowMap& c = self.cast<ColorFlowMap&>();
+                               return view<real>(c.power.data(), {c.grid.n_axis0, c.grid.n_axis1}, self);
+                             })
+      .def_property_readonly("variance",
+                             [](py::object self) {
+                               ColorFlowMap& c = self.cast<ColorFlowMap&>();
+                               return view<real>(c.variance.data(), {c.grid.n_axis0, c.grid.n_axis1}, self);
+                             })
+      .def_property_readonly("valid", [](py::object self) {
+        ColorFlowMap& c = self.cast<ColorFlowMap&>();
+        return view<std::uint8_t>(c.valid.data(), {c.grid.n_axis0, c.grid.n_axis1}, self);
+      });
+
+  m.def("color_flow", &color_flow, py::arg("ensemble"), py::arg("prf"), py::arg("speed_of_sound"),
+        py::arg("config") = DopplerConfig{}, py::call_guard<py::gil_scoped_release>());
+  m.def("apply_wall_filter", &apply_wall_filter, py::arg("ensemble"), py::arg("type"),
+        py::arg("polynomial_order") = 1, py::call_guard<py::gil_scoped_release>());
+  m.def(
+      "power_doppler",
+      [](const std::vector<Frame>& ens, WallFilter type, int order) {
+        std::vector<real> p;
+        {
+          py::gil_scoped_release release;
+          p = power_doppler(ens, type, order);
+        }
+        if (ens.empty()) return copy_vector(p, {0});
+        return copy_vector(p, {ens.front().grid.n_axis0, ens.front().grid.n_axis1});
+      },
+      py::arg("ensemble"), py::arg("type") = WallFilter::PolynomialRegression,
+      py::arg("polynomial_order") = 1);
+  m.def(
+      "polynomial_reject_matrix",
+      [](int n, int order) { return copy_vector(polynomial_reject_matrix(n, order), {n, n}); },
+      py::arg("n"), py::arg("order"));
+
+  py::class_<DisplacementMap>(m, "DisplacementMap")
+      .def_readonly("grid", &DisplacementMap::grid)
+      .def_property_readonly("axial",
+                             [](py::object self) {
+                               DisplacementMap& d = self.cast<DisplacementMap&>();
+                               return view<real>(d.axial.data(), {d.grid.n_axis0, d.grid.n_axis1}, self);
+                             })
+      .def_property_readonly("correlation", [](py::object self) {
+        DisplacementMap& d = self.cast<DisplacementMap&>();
+        return view<real>(d.correlation.data(), {d.grid.n_axis0, d.grid.n_axis1}, self);
+      });
+
+  m.def("estimate_axial_displacement", &estimate_axial_displacement, py::arg("before"),
+        py::arg("after"), py::arg("speed_of_sound"), py::arg("kernel_axial") = 8,
+        py::arg("kernel_lateral") = 2, py::arg("max_lag") = 8,
+        py::call_guard<py::gil_scoped_release>());
+  m.def(
+      "axial_strain",
+      [](const DisplacementMap& d, int fit_length) {
+        std::vector<real> st;
+        {
+          py::gil_scoped_release release;
+          st = axial_strain(d, fit_length);
+        }
+        return copy_vector(st, {d.grid.n_axis0, d.grid.n_axis1});
+      },
+      py::arg("displacement"), py::arg("fit_length") = 15);
+
+  py::class_<DopplerRunner>(m, "DopplerRunner")
+      .def(py::init<const ScanGrid&, int, real, real, real, const DopplerConfig&>(),
+           py::arg("grid"), py::arg("n_frames"), py::arg("prf"), py::arg("speed_of_sound"),
+           py::arg("demodulation_frequency"), py::arg("config") = DopplerConfig{})
+      .def("color_flow", &DopplerRunner::color_flow, py::arg("ensemble_ptr"),
+           py::arg("velocity_ptr"), py::arg("power_ptr"), py::arg("variance_ptr"),
+           py::arg("valid_ptr"))
+      .def("power_doppler", &DopplerRunner::power_doppler, py::arg("ensemble_ptr"),
+           py::arg("power_ptr"))
+      .def("wall_filter", &DopplerRunner::wall_filter, py::arg("ensemble_ptr"), py::arg("out_ptr"))
+      .def("synchronize", &DopplerRunner::synchronize, py::call_guard<py::gil_scoped_release>())
+      .def("record", &DopplerRunner::record, py::arg("slot"))
+      .def("elapsed", &DopplerRunner::elapsed, py::arg("a"), py::arg("b"))
+      .def("profile", &DopplerRunner::profile, py::arg("on"))
+      .def("profile_report", &DopplerRunner::profile_report)
+      .def_property_readonly("n_frames", &DopplerRunner::n_frames);
+
   m.attr("APOD_PARAMS") = USX_APOD_PARAMS;
 
   m.def("build_info", []() { return std::string(usx_build_info()); });

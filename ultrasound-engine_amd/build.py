@@ -30,8 +30,9 @@ ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "usx")
 BUILD = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["context.hip", "demod.hip", "beamform.hip", "mv.hip", "apod.hip", "sim.hip", "display.hip"]
-CORE_SOURCES = ["types.cpp", "geometry.cpp", "gpu.cpp", "beamformer.cpp", "pipeline.cpp",
+HIP_SOURCES = ["context.hip", "demod.hip", "beamform.hip", "mv.hip", "apod.hip", "sim.hip", "display.hip",
+               "doppler.hip"]
+CORE_SOURCES = ["types.cpp", "geometry.cpp", "gpu.cpp", "beamformer.cpp", "pipeline.cpp", "doppler.cpp",
                 "postprocess.cpp", "simulator.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -73,6 +73,9 @@ struct usx_ctx {
   DevBuf s_scat, s_live, s_t0, s_n, s_pos, s_wave, s_partial, s_bmax, s_rf;
   // display tail
   DevBuf disp_env, disp_bmax, disp_img, disp_frame;
+  // Doppler and speckle tracking
+  Cached dop_reject;
+  DevBuf dop_in, dop_in2, dop_out, dop_lag, dop_f0, dop_f1, dop_f2, dop_valid, dop_pmax;
 };
 
 namespace {
@@ -591,10 +594,13 @@ usx_status usx_ctx_destroy(usx_ctx* c) {
                     &c->tr_db, &c->tr_loss_part, &c->tr_losses, &c->tr_grads, &c->mv_w,
                     &c->mv_status, &c->ap_pixels, &c->s_scat, &c->s_live, &c->s_t0, &c->s_n,
                     &c->s_pos, &c->s_wave, &c->s_partial, &c->s_bmax, &c->s_rf, &c->disp_env,
-                    &c->disp_bmax, &c->disp_img, &c->disp_frame})
+                    &c->disp_bmax, &c->disp_img, &c->disp_frame, &c->dop_in, &c->dop_in2,
+                    &c->dop_out, &c->dop_lag, &c->dop_f0, &c->dop_f1, &c->dop_f2, &c->dop_valid,
+                    &c->dop_pmax})
     if (b->p) (void)hipFree(b->p);
   for (Cached* t : {&c->taps, &c->rot, &c->ex, &c->tx, &c->ax0, &c->ax1, &c->s0, &c->c0, &c->s_ex,
-                    &c->s_delays, &c->s_firing, &c->s_fw, &c->s_nf, &c->s_pulse, &c->s_noise})
+                    &c->s_delays, &c->s_firing, &c->s_fw, &c->s_nf, &c->s_pulse, &c->s_noise,
+                    &c->dop_reject})
     if (t->dev.p) (void)hipFree(t->dev.p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1495,6 +1501,177 @@ usx_status usx_process_display(usx_ctx* c, const usx_demod_desc* d, const usx_bf
       t->postprocess_ms = cc;
       t->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     }
+  });
+}
+
+}  // extern "C"
+
+// ---- Doppler and speckle tracking (doppler.cpp) ---------------------------
+
+namespace {
+
+size_t ensemble_pixels(usx_ctx* c, const usx_ensemble_desc* e, int min_frames, const char* who) {
+  if (!e) fail(c, USX_ERR_INVALID, "%s: null descriptor", who);
+  if (e->n_frames < min_frames)
+    fail(c, USX_ERR_INVALID, "%s: need at least %d frames in the ensemble", who, min_frames);
+  if (e->n_axis0 <= 0 || e->n_axis1 <= 0) fail(c, USX_ERR_INVALID, "ScanGrid dimensions must be > 0");
+  return static_cast<size_t>(e->n_axis0) * e->n_axis1;
+}
+
+// The projector on the device, or nullptr for no filter.
+const float* reject_on_device(usx_ctx* c, const usx_ensemble_desc* e) {
+  if (!e->reject) return nullptr;
+  const size_t bytes = sizeof(float) * static_cast<size_t>(e->n_frames) * e->n_frames;
+  return static_cast<const float*>(upload(c, c->dop_reject, e->reject, bytes));
+}
+
+// The ensemble on the device: the caller's pointer, or an upload of it.
+const float2* ensemble_on_device(usx_ctx* c, const usx_ensemble_desc* e, const usx_c32* ens,
+                                 int memory, DevBuf& buf) {
+  if (memory == USX_DEVICE) return reinterpret_cast<const float2*>(ens);
+  const size_t bytes = sizeof(float2) * e->n_frames * ensemble_pixels(c, e, 1, "ensemble");
+  void* d = ensure(c, buf, bytes);
+  hip_check(c, hipMemcpyAsync(d, ens, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(ensemble)");
+  return static_cast<const float2*>(d);
+}
+
+template <typename T>
+T* out_on_device(usx_ctx* c, T* host_or_dev, size_t n, int memory, DevBuf& buf) {
+  if (memory == USX_DEVICE) return host_or_dev;
+  return static_cast<T*>(ensure(c, buf, n * sizeof(T)));
+}
+
+template <typename T>
+void download(usx_ctx* c, T* host, const T* dev, size_t n, int memory, const char* what) {
+  if (memory == USX_DEVICE) return;
+  hip_check(c, hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, c->stream), what);
+}
+
+void finish(usx_ctx* c, int memory, const char* what) {
+  check_launch(c, what);
+  if (memory != USX_DEVICE) hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+
+extern "C" {
+
+usx_status usx_wall_filter(usx_ctx* c, const usx_ensemble_desc* e, const usx_c32* ensemble,
+                           usx_c32* out, int memory) {
+  return guarded(c, [&] {
+    const size_t n_pix = ensemble_pixels(c, e, 1, "apply_wall_filter");
+    if (!ensemble || !out) fail(c, USX_ERR_INVALID, "apply_wall_filter: null data pointer");
+    const size_t n = n_pix * e->n_frames;
+    const float* M = reject_on_device(c, e);
+    const float2* d_in = ensemble_on_device(c, e, ensemble, memory, c->dop_in);
+    float2* d_out = out_on_device(c, reinterpret_cast<float2*>(out), n, memory, c->dop_out);
+    timed(c, "wall_filter", [&] { usxd::launch_wall_filter(e->n_frames, n_pix, d_in, M, d_out, c->stream); });
+    download(c, reinterpret_cast<float2*>(out), d_out, n, memory, "hipMemcpyAsync(filtered)");
+    finish(c, memory, "wall filter launch");
+  });
+}
+
+usx_status usx_power_doppler(usx_ctx* c, const usx_ensemble_desc* e, const usx_c32* ensemble,
+                             float* power, int memory) {
+  return guarded(c, [&] {
+    const size_t n_pix = ensemble_pixels(c, e, 1, "power_doppler");
+    if (!ensemble || !power) fail(c, USX_ERR_INVALID, "power_doppler: null data pointer");
+    const float* M = reject_on_device(c, e);
+    const float2* d_in = ensemble_on_device(c, e, ensemble, memory, c->dop_in);
+    float* d_p = out_on_device(c, power, n_pix, memory, c->dop_f0);
+    timed(c, "power_doppler", [&] { usxd::launch_power_doppler(e->n_frames, n_pix, d_in, M, d_p, c->stream); });
+    download(c, power, d_p, n_pix, memory, "hipMemcpyAsync(power)");
+    finish(c, memory, "power doppler launch");
+  });
+}
+
+usx_status usx_color_flow(usx_ctx* c, const usx_ensemble_desc* e, const usx_color_flow_desc* f,
+                          const usx_c32* ensemble, float* velocity, float* power, float* variance,
+                          uint8_t* valid, int memory) {
+  return guarded(c, [&] {
+    const size_t n_pix = ensemble_pixels(c, e, 2, "color_flow");
+    if (!f) fail(c, USX_ERR_INVALID, "color_flow: null descriptor");
+    if (!ensemble || !velocity || !power || !variance || !valid)
+      fail(c, USX_ERR_INVALID, "color_flow: null data pointer");
+    usxd::ColorFlowArgs A;
+    A.n = e->n_frames;
+    A.n0 = e->n_axis0;
+    A.n1 = e->n_axis1;
+    A.ka = std::max(0, f->kernel_axial);
+    A.kl = std::max(0, f->kernel_lateral);
+    A.loupas = f->loupas;
+    A.fdem = f->demodulation_frequency;
+    A.c = f->speed_of_sound;
+    A.prf = f->prf;
+    A.dz = f->dz;
+    A.coherence_threshold = f->coherence_threshold;
+    const float* M = reject_on_device(c, e);
+    const float2* d_in = ensemble_on_device(c, e, ensemble, memory, c->dop_in);
+    float* lag = static_cast<float*>(ensure(c, c->dop_lag, 5 * n_pix * sizeof(float)));
+    unsigned int* pmax = static_cast<unsigned int*>(ensure(c, c->dop_pmax, 16));
+    float* d_v = out_on_device(c, velocity, n_pix, memory, c->dop_f0);
+    float* d_p = out_on_device(c, power, n_pix, memory, c->dop_f1);
+    float* d_w = out_on_device(c, variance, n_pix, memory, c->dop_f2);
+    uint8_t* d_ok = out_on_device(c, valid, n_pix, memory, c->dop_valid);
+    timed(c, "color_flow", [&] {
+      usxd::launch_color_flow(A, d_in, M, lag, d_v, d_p, d_w, d_ok, pmax, f->power_factor, c->stream);
+    });
+    download(c, velocity, d_v, n_pix, memory, "hipMemcpyAsync(velocity)");
+    download(c, power, d_p, n_pix, memory, "hipMemcpyAsync(power)");
+    download(c, variance, d_w, n_pix, memory, "hipMemcpyAsync(variance)");
+    download(c, valid, d_ok, n_pix, memory, "hipMemcpyAsync(valid)");
+    finish(c, memory, "color flow launch");
+  });
+}
+
+usx_status usx_axial_displacement(usx_ctx* c, const usx_displacement_desc* d,
+                                  const usx_c32* before, const usx_c32* after, float* axial,
+                                  float* correlation, int memory) {
+  return guarded(c, [&] {
+    if (!d) fail(c, USX_ERR_INVALID, "estimate_axial_displacement: null descriptor");
+    if (d->n_axis0 <= 0 || d->n_axis1 <= 0) fail(c, USX_ERR_INVALID, "ScanGrid dimensions must be > 0");
+    if (!before || !after || !axial || !correlation)
+      fail(c, USX_ERR_INVALID, "estimate_axial_displacement: null data pointer");
+    const size_t n_pix = static_cast<size_t>(d->n_axis0) * d->n_axis1;
+    usxd::DisplacementArgs A;
+    A.n0 = d->n_axis0;
+    A.n1 = d->n_axis1;
+    A.ka = std::max(1, d->kernel_axial);
+    A.kl = std::max(0, d->kernel_lateral);
+    A.lag_max = std::max(0, d->max_lag);
+    A.fdem = d->demodulation_frequency;
+    A.c = d->speed_of_sound;
+    A.dz = d->dz;
+    const usx_ensemble_desc one{1, d->n_axis0, d->n_axis1, nullptr};
+    const float2* d_a = ensemble_on_device(c, &one, before, memory, c->dop_in);
+    const float2* d_b = ensemble_on_device(c, &one, after, memory, c->dop_in2);
+    float* d_ax = out_on_device(c, axial, n_pix, memory, c->dop_f0);
+    float* d_co = out_on_device(c, correlation, n_pix, memory, c->dop_f1);
+    timed(c, "displacement", [&] { usxd::launch_displacement(A, d_a, d_b, d_ax, d_co, c->stream); });
+    download(c, axial, d_ax, n_pix, memory, "hipMemcpyAsync(axial)");
+    download(c, correlation, d_co, n_pix, memory, "hipMemcpyAsync(correlation)");
+    finish(c, memory, "displacement launch");
+  });
+}
+
+usx_status usx_axial_strain(usx_ctx* c, int n_axis0, int n_axis1, float dz, int fit_length,
+                            const float* axial, float* strain, int memory) {
+  return guarded(c, [&] {
+    if (n_axis0 <= 0 || n_axis1 <= 0) fail(c, USX_ERR_INVALID, "ScanGrid dimensions must be > 0");
+    if (!axial || !strain) fail(c, USX_ERR_INVALID, "axial_strain: null data pointer");
+    const size_t n_pix = static_cast<size_t>(n_axis0) * n_axis1;
+    const float* d_ax = axial;
+    if (memory != USX_DEVICE) {
+      float* q = static_cast<float*>(ensure(c, c->dop_f1, n_pix * sizeof(float)));
+      hip_check(c, hipMemcpyAsync(q, axial, n_pix * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                "hipMemcpyAsync(axial)");
+      d_ax = q;
+    }
+    float* d_s = out_on_device(c, strain, n_pix, memory, c->dop_f0);
+    const int half = std::max(1, fit_length / 2);
+    timed(c, "strain", [&] { usxd::launch_strain(n_axis0, n_axis1, dz, half, d_ax, d_s, c->stream); });
+    download(c, strain, d_s, n_pix, memory, "hipMemcpyAsync(strain)");
+    finish(c, memory, "strain launch");
   });
 }
 

@@ -179,4 +179,33 @@ void launch_display_envelope(const DisplayArgs& D, const float2* frame, bool rf,
 void launch_display_compress(const DisplayArgs& D, const float* env, const float* blockmax,
                              int n_blocks, const float* ref, float* image, hipStream_t st);
 
+// doppler.hip: clutter filter, colour flow, power Doppler, axial displacement
+// and strain on beamformed frames. Ensembles are frame-major, n frames of
+// n_pix complex pixels; M is the row-major n x n wall-filter projector or
+// nullptr (no filter).
+struct ColorFlowArgs {
+  int n, n0, n1;
+  int ka, kl;              // window half sizes (axial, lateral), >= 0
+  int loupas;
+  float fdem, c, prf, dz;
+  float coherence_threshold;
+};
+struct DisplacementArgs {
+  int n0, n1;
+  int ka, kl, lag_max;     // ka >= 1, kl >= 0, lag_max >= 0
+  float fdem, c, dz;
+};
+void launch_wall_filter(int n, size_t n_pix, const float2* in, const float* M, float2* out,
+                        hipStream_t st);
+void launch_power_doppler(int n, size_t n_pix, const float2* in, const float* M, float* p,
+                          hipStream_t st);
+// lag: 5 * n_pix floats of scratch; pmax_bits: one word of scratch.
+void launch_color_flow(const ColorFlowArgs& A, const float2* in, const float* M, float* lag,
+                       float* velocity, float* power, float* variance, uint8_t* valid,
+                       unsigned int* pmax_bits, float factor, hipStream_t st);
+void launch_displacement(const DisplacementArgs& A, const float2* before, const float2* after,
+                         float* axial, float* correlation, hipStream_t st);
+void launch_strain(int n0, int n1, float dz, int half, const float* axial, float* strain,
+                   hipStream_t st);
+
 }  // namespace usxd

@@ -26,10 +26,14 @@ from ._usx import (  # noqa: F401
     Beamformer,
     BeamformerConfig,
     ChannelData,
+    ColorFlowMap,
     Combiner,
     Compounding,
     DemodConfig,
     DeviceRunner,
+    DisplacementMap,
+    DopplerConfig,
+    DopplerRunner,
     Frame,
     GridType,
     ImagingPipeline,
@@ -46,17 +50,24 @@ from ._usx import (  # noqa: F401
     Transmit,
     TransmitType,
     Vec3,
+    WallFilter,
     Window,
     __version__,
     apply_tgc,
+    apply_wall_filter,
+    axial_strain,
     build_info,
+    color_flow,
     compute_transmit_delays,
     demodulate,
     design_lowpass,
     dynamic_aperture_weight,
     element_directivity,
     envelope,
+    estimate_axial_displacement,
     log_compress,
+    polynomial_reject_matrix,
+    power_doppler,
     transmit_arrival_time,
     transmit_sensitivity,
     window_value,
