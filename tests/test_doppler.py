@@ -146,11 +146,16 @@ def test_color_flow_matches_the_oracle_on_simulated_flow(usx_mod, O, port, flow,
 
 
 @pytest.mark.parametrize("n", [2, 17, 40])
-def test_color_flow_matches_the_oracle_on_synthetic_ensembles(usx_mod, O, port, flow, n):
+@pytest.mark.parametrize("ka,kl", [(4, 1), (20, 1), (2, 6)])
+def test_color_flow_matches_the_oracle_on_synthetic_ensembles(usx_mod, O, port, flow, n, ka, kl):
+    """Windows beyond the LDS tile (axial > 16 or lateral > 4) take the
+    direct-from-memory path."""
     frames, data = _synthetic(usx_mod, flow, n, seed=100 + n)
     dc = usx_mod.DopplerConfig()
     dc.coherence_threshold = 0.2
     dc.power_threshold_db = -10.0
+    dc.kernel_axial = ka
+    dc.kernel_lateral = kl
     _compare_flow(usx_mod, O, port, frames, data, flow[1], 5000.0, dc)
 
 
@@ -248,3 +253,55 @@ def test_errors(usx_mod, flow):
     with pytest.raises(ValueError):
         usx_mod.estimate_axial_displacement(ens[0], other[0], C)
     assert usx_mod.apply_wall_filter([], usx_mod.WallFilter.MEAN, 0) == []
+
+
+def test_displacement_and_strain_physics(usx_mod):
+    """core/tests/test_main.cpp:506-600 restated: a 20 um translation is
+    tracked within 2 um, a 400 um one only with the lag search, a uniform
+    translation has no strain and a 0.5 % compression has that strain."""
+    usx = usx_mod
+    probe, med, acq, _ = _standard(usx, n_samples=2600, prf=4000.0)
+    cfg = usx.SimulatorConfig()
+    cfg.noise_db = -200.0
+    cfg.enable_motion = False
+    sim = usx.Simulator(probe, med, acq, cfg)
+    before = usx.Phantom.speckle_box(-0.006, 0.006, 0.026, 0.034, 1e-4, 6000, 11)
+    txs = sim.make_plane_waves([0.0])
+    grid = usx.ScanGrid.cartesian(-0.004, 0.004, 41, 0.028, 0.032, 121)
+    bf = usx.Beamformer()
+    pos = np.asarray(before.positions, np.float64)
+
+    def frame_of(z_new):
+        ph = before.copy()
+        p = pos.copy()
+        p[:, 2] = z_new
+        ph.positions = p.astype(np.float32)
+        return bf.beamform(usx.demodulate(sim.simulate(txs, ph), usx.DemodConfig()), grid)
+
+    fa = bf.beamform(usx.demodulate(sim.simulate(txs, before), usx.DemodConfig()), grid)
+    fb = frame_of(np.float32(pos[:, 2]) + np.float32(20e-6))
+    dm = usx.estimate_axial_displacement(fa, fb, med.speed_of_sound, 8, 2)
+    good = np.asarray(dm.correlation) >= 0.9
+    assert good.sum() > 100
+    assert float(np.mean(np.asarray(dm.axial)[good])) * 1e6 == pytest.approx(20.0, abs=2.0)
+
+    ff = frame_of(np.float32(pos[:, 2]) + np.float32(400e-6))
+
+    def median_of(m):
+        ok = np.asarray(m.correlation) > 0.8
+        return float(np.median(np.asarray(m.axial)[ok])) if ok.any() else 0.0
+
+    m_phase = median_of(usx.estimate_axial_displacement(fa, ff, med.speed_of_sound, 8, 2, 0))
+    m_lag = median_of(usx.estimate_axial_displacement(fa, ff, med.speed_of_sound, 8, 2, 12))
+    assert abs(m_lag - 400e-6) < 20e-6
+    assert abs(m_phase - 400e-6) > 50e-6
+
+    strain = np.asarray(usx.axial_strain(dm, 15))
+    assert abs(float(np.mean(strain[:, 20:-20]))) < 5e-3
+
+    z = pos[:, 2].astype(np.float32)
+    fc = frame_of(z + np.float32(-0.005) * (z - np.float32(0.030)))
+    dm2 = usx.estimate_axial_displacement(fa, fc, med.speed_of_sound, 8, 2)
+    s2 = np.asarray(usx.axial_strain(dm2, 21))[:, 25:-25]
+    ok = np.asarray(dm2.correlation)[:, 25:-25] >= 0.9
+    assert float(np.mean(s2[ok])) == pytest.approx(-0.005, abs=1.5e-3)

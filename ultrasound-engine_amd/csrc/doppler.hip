@@ -16,8 +16,8 @@
 // (2 kl + 1) x (2 ka + 1) window and over the ensemble in one running sum.
 // Here the ensemble sums of each pixel (zero lag power, slow-time lag one,
 // axial lag one) are formed once in the first pass, in the reference's
-// ensemble order, and the second pass adds those over the window in the
-// reference's window order: the same terms, regrouped (float rounding only).
+// ensemble order, and the second pass adds those over the window, depth
+// first then line: the same terms, regrouped (float rounding only).
 // The wall filter is applied in registers in the first pass, so the filtered
 // ensemble is never written for colour flow or power Doppler.
 #include <hip/hip_runtime.h>
@@ -139,27 +139,38 @@ __global__ __launch_bounds__(256) void power_doppler_kernel(int n, size_t n_pix,
 //   lag[0][k] = sum_e |z_e|^2
 //   lag[1..2][k] = sum_{e < n-1} conj(z_e) z_{e+1}       (slow time)
 //   lag[3..4][k] = sum_e conj(z_e[k]) z_e[k + 1]           (axial; 0 on the last depth)
-// (doppler.cpp:119-130 with the window loop taken out).
+// (doppler.cpp:119-130 with the window loop taken out). A wavefront loads and
+// filters 64 consecutive pixels and writes the first 63: the axial neighbour's
+// filtered ensemble comes from the next lane by a shuffle, so every pixel is
+// filtered once. Thread 0 of the grid also clears the power maximum.
 // ---------------------------------------------------------------------------
+constexpr int kLagOut = 63;  // pixels written per wavefront
+
 template <int NC>
 __global__ __launch_bounds__(256) void cf_lag_kernel(int n, int n0, int n1,
                                                      const float2* __restrict__ in,
                                                      const float* __restrict__ M,
-                                                     float* __restrict__ lag) {
+                                                     float* __restrict__ lag,
+                                                     unsigned int* __restrict__ pmax_bits) {
   const size_t n_pix = static_cast<size_t>(n0) * n1;
-  const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= n_pix) return;
-  const int j = static_cast<int>(k % static_cast<size_t>(n1));
+  const int lane = threadIdx.x & 63;
+  const size_t wave = static_cast<size_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const size_t k = wave * kLagOut + lane;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *pmax_bits = 0u;
+  if (wave * kLagOut >= n_pix) return;  // wavefront-uniform
+  const bool live = k < n_pix;
+  const size_t kc = live ? k : n_pix - 1;  // clamped: every lane takes part in the shuffles
+  const int j = static_cast<int>(kc % static_cast<size_t>(n1));
   const bool axial = j + 1 < n1;
   float r00 = 0.0f;
   float2 r01 = make_float2(0.0f, 0.0f), r10 = make_float2(0.0f, 0.0f);
   if constexpr (NC > 0) {
-    float2 z[NC], zn[NC];
-    load_filtered<NC>(in, n_pix, k, n, M, z);
-    if (axial) load_filtered<NC>(in, n_pix, k + 1, n, M, zn);
+    float2 z[NC];
+    load_filtered<NC>(in, n_pix, kc, n, M, z);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       if (e < n) {
+        const float2 zn = make_float2(__shfl_down(z[e].x, 1, 64), __shfl_down(z[e].y, 1, 64));
         r00 += cnorm(z[e]);
         if (e + 1 < n) {
           const float2 p = cmul_conj(z[e], z[e + 1 < NC ? e + 1 : e]);
@@ -167,31 +178,33 @@ __global__ __launch_bounds__(256) void cf_lag_kernel(int n, int n0, int n1,
           r01.y += p.y;
         }
         if (axial) {
-          const float2 q = cmul_conj(z[e], zn[e]);
+          const float2 q = cmul_conj(z[e], zn);
           r10.x += q.x;
           r10.y += q.y;
         }
       }
     }
   } else {
-    float2 cur = filtered_at(in, n_pix, k, n, M, 0);
+    float2 cur = filtered_at(in, n_pix, kc, n, M, 0);
     for (int e = 0; e < n; ++e) {
+      const float2 zn = make_float2(__shfl_down(cur.x, 1, 64), __shfl_down(cur.y, 1, 64));
       r00 += cnorm(cur);
       float2 nxt = make_float2(0.0f, 0.0f);
       if (e + 1 < n) {
-        nxt = filtered_at(in, n_pix, k, n, M, e + 1);
+        nxt = filtered_at(in, n_pix, kc, n, M, e + 1);
         const float2 p = cmul_conj(cur, nxt);
         r01.x += p.x;
         r01.y += p.y;
       }
       if (axial) {
-        const float2 q = cmul_conj(cur, filtered_at(in, n_pix, k + 1, n, M, e));
+        const float2 q = cmul_conj(cur, zn);
         r10.x += q.x;
         r10.y += q.y;
       }
       cur = nxt;
     }
   }
+  if (!live || lane == kLagOut) return;
   lag[k] = r00;
   lag[n_pix + k] = r01.x;
   lag[2 * n_pix + k] = r01.y;
@@ -200,17 +213,114 @@ __global__ __launch_bounds__(256) void cf_lag_kernel(int n, int n0, int n1,
 }
 
 // ---------------------------------------------------------------------------
-// Colour flow, pass 2: window sums (di outer, dj inner, as doppler.cpp:111-131)
-// and the estimates (doppler.cpp:133-161). valid is set before the power
-// threshold, and the largest finite power goes to *pmax_bits (non-negative
-// floats order as their bit patterns).
+// Colour flow, pass 2: window sums and the estimates (doppler.cpp:111-161).
+// A workgroup owns kCfLines lines x 64 depths; it stages the five lag maps of
+// the tile and its halo (kl lines, ka depths either side; zeros off the
+// grid, which add nothing) in LDS, sums each staged row over the 2 ka + 1
+// depths, then each pixel over the 2 kl + 1 rows: the reference's window sum
+// regrouped as depth-then-line (float rounding only). valid is set before the
+// power threshold; the largest finite power goes to *pmax_bits (non-negative
+// floats order as their bit patterns) by one atomic per workgroup.
 // ---------------------------------------------------------------------------
+constexpr int kCfLines = 4;
+constexpr int kCfMaxKa = 16, kCfMaxKl = 4;
+constexpr int kCfRows = kCfLines + 2 * kCfMaxKl;
+constexpr int kCfCols = 64 + 2 * kCfMaxKa;
+
+__device__ __forceinline__ void cf_estimate_pixel(const ColorFlowArgs& A, float R00, float2 R01,
+                                                  float2 R10, float* v_out, float* var_out,
+                                                  uint8_t* ok_out) {
+  float v = 0.0f, var = 0.0f;
+  uint8_t ok = 0;
+  if (R00 > 0.0f) {
+    const float coherence = hypotf(R01.x, R01.y) / R00;
+    var = clamp01(2.0f * (1.0f - coherence));
+    if (!(coherence < A.coherence_threshold)) {
+      const float phase_slow = atan2f(R01.y, R01.x);
+      float f_centre = A.fdem;
+      if (A.loupas) {
+        const float phase_fast = atan2f(R10.y, R10.x);
+        const float f_est = phase_fast * A.c / (4.0f * kPiF * A.dz);
+        if (f_est > 0.25f * A.fdem && f_est < 2.0f * A.fdem) f_centre = f_est;
+      }
+      if (!(f_centre <= 0.0f)) {
+        v = -phase_slow * A.c * A.prf / (4.0f * kPiF * f_centre);
+        ok = 1;
+      }
+    }
+  }
+  *v_out = v;
+  *var_out = var;
+  *ok_out = ok;
+}
+
 __global__ __launch_bounds__(256) void cf_estimate_kernel(ColorFlowArgs A, const float* __restrict__ lag,
                                                           float* __restrict__ velocity,
                                                           float* __restrict__ power,
                                                           float* __restrict__ variance,
                                                           uint8_t* __restrict__ valid,
                                                           unsigned int* pmax_bits) {
+  __shared__ float stage[5][kCfRows][kCfCols];
+  __shared__ float rows[5][kCfRows][64];
+  __shared__ unsigned int wave_max[4];
+  const size_t n_pix = static_cast<size_t>(A.n0) * A.n1;
+  const int tid = threadIdx.x;
+  const int n_jt = (A.n1 + 63) / 64;
+  const int i0 = (blockIdx.x / n_jt) * kCfLines, j0 = (blockIdx.x % n_jt) * 64;
+  const int nr = kCfLines + 2 * A.kl, nc = 64 + 2 * A.ka;
+  for (int q = tid; q < nr * nc; q += 256) {
+    const int r = q / nc, c = q - r * nc;
+    const int ii = i0 - A.kl + r, jj = j0 - A.ka + c;
+    const bool in = ii >= 0 && ii < A.n0 && jj >= 0 && jj < A.n1;
+    const size_t kk = in ? static_cast<size_t>(ii) * A.n1 + jj : 0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) stage[m][r][c] = in ? lag[m * n_pix + kk] : 0.0f;
+  }
+  __syncthreads();
+  for (int q = tid; q < nr * 64; q += 256) {
+    const int r = q >> 6, c = q & 63;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      float s = 0.0f;
+      for (int d = 0; d <= 2 * A.ka; ++d) s += stage[m][r][c + d];
+      rows[m][r][c] = s;
+    }
+  }
+  __syncthreads();
+  const int li = tid >> 6, c = tid & 63;
+  const int i = i0 + li, j = j0 + c;
+  const bool live = i < A.n0 && j < A.n1;
+  float R[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (int d = 0; d <= 2 * A.kl; ++d) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) R[m] += rows[m][li + d][c];
+  }
+  float pw = 0.0f;
+  if (live) {
+    const size_t k = static_cast<size_t>(i) * A.n1 + j;
+    cf_estimate_pixel(A, R[0], make_float2(R[1], R[2]), make_float2(R[3], R[4]), velocity + k,
+                      variance + k, valid + k);
+    power[k] = R[0];
+    pw = R[0] == R[0] ? R[0] : 0.0f;
+  }
+  unsigned int mx = __float_as_uint(pw);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, static_cast<unsigned int>(__shfl_xor(static_cast<int>(mx), o, 64)));
+  if ((tid & 63) == 0) wave_max[tid >> 6] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned int b = max(max(wave_max[0], wave_max[1]), max(wave_max[2], wave_max[3]));
+    if (b) atomicMax(pmax_bits, b);
+  }
+}
+
+// Window sums straight from memory, for windows larger than the LDS tile.
+__global__ __launch_bounds__(256) void cf_estimate_global_kernel(ColorFlowArgs A, const float* __restrict__ lag,
+                                                                 float* __restrict__ velocity,
+                                                                 float* __restrict__ power,
+                                                                 float* __restrict__ variance,
+                                                                 uint8_t* __restrict__ valid,
+                                                                 unsigned int* pmax_bits) {
   const size_t n_pix = static_cast<size_t>(A.n0) * A.n1;
   const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= n_pix) return;
@@ -232,29 +342,8 @@ __global__ __launch_bounds__(256) void cf_estimate_kernel(ColorFlowArgs A, const
       R10.y += lag[4 * n_pix + kk];
     }
   }
-  float v = 0.0f, var = 0.0f;
-  uint8_t ok = 0;
-  if (R00 > 0.0f) {
-    const float coherence = hypotf(R01.x, R01.y) / R00;
-    var = clamp01(2.0f * (1.0f - coherence));
-    if (!(coherence < A.coherence_threshold)) {
-      const float phase_slow = atan2f(R01.y, R01.x);
-      float f_centre = A.fdem;
-      if (A.loupas) {
-        const float phase_fast = atan2f(R10.y, R10.x);
-        const float f_est = phase_fast * A.c / (4.0f * kPiF * A.dz);
-        if (f_est > 0.25f * A.fdem && f_est < 2.0f * A.fdem) f_centre = f_est;
-      }
-      if (!(f_centre <= 0.0f)) {
-        v = -phase_slow * A.c * A.prf / (4.0f * kPiF * f_centre);
-        ok = 1;
-      }
-    }
-  }
-  velocity[k] = v;
+  cf_estimate_pixel(A, R00, R01, R10, velocity + k, variance + k, valid + k);
   power[k] = R00;
-  variance[k] = var;
-  valid[k] = ok;
   if (R00 == R00) atomicMax(pmax_bits, __float_as_uint(R00));
 }
 
@@ -407,10 +496,18 @@ void launch_color_flow(const ColorFlowArgs& A, const float2* in, const float* M,
                        float* velocity, float* power, float* variance, uint8_t* valid,
                        unsigned int* pmax_bits, float factor, hipStream_t st) {
   const size_t n_pix = static_cast<size_t>(A.n0) * A.n1;
-  USX_ENS_DISPATCH(cf_lag_kernel, A.n, pix_grid(n_pix), dim3(256), 0, st, A.n, A.n0, A.n1, in, M, lag);
-  launch_fill_zero(reinterpret_cast<float*>(pmax_bits), 1, st);
-  hipLaunchKernelGGL(cf_estimate_kernel, pix_grid(n_pix), dim3(256), 0, st, A, lag, velocity, power,
-                     variance, valid, pmax_bits);
+  const size_t waves = (n_pix + kLagOut - 1) / kLagOut;
+  const dim3 lag_grid(static_cast<unsigned>((waves + 3) / 4));
+  USX_ENS_DISPATCH(cf_lag_kernel, A.n, lag_grid, dim3(256), 0, st, A.n, A.n0, A.n1, in, M, lag,
+                   pmax_bits);
+  if (A.ka <= kCfMaxKa && A.kl <= kCfMaxKl) {
+    const unsigned tiles = static_cast<unsigned>(((A.n0 + kCfLines - 1) / kCfLines) * ((A.n1 + 63) / 64));
+    hipLaunchKernelGGL(cf_estimate_kernel, dim3(tiles), dim3(256), 0, st, A, lag, velocity, power,
+                       variance, valid, pmax_bits);
+  } else {
+    hipLaunchKernelGGL(cf_estimate_global_kernel, pix_grid(n_pix), dim3(256), 0, st, A, lag, velocity,
+                       power, variance, valid, pmax_bits);
+  }
   hipLaunchKernelGGL(cf_threshold_kernel, pix_grid(n_pix), dim3(256), 0, st, n_pix, power, valid,
                      pmax_bits, factor);
 }

@@ -1,6 +1,8 @@
-"""Standard test phantoms (reference: python/usx/phantoms.py:17-73)."""
+"""Standard test phantoms (reference: python/usx/phantoms.py:17-143)."""
 
 from __future__ import annotations
+
+import numpy as np
 
 from ._usx import Medium, Phantom, Probe, Vec3
 
@@ -43,3 +45,54 @@ def resolution_and_contrast(probe: Probe, medium: Medium | None = None,
     ph.extend(Phantom.point_targets(
         [Vec3(-0.008, 0.0, float(z)) for z in (0.012, 0.024, 0.036, 0.048)], 40.0))
     return ph
+
+
+def flow_phantom(probe: Probe, medium: Medium | None = None,
+                 vessel_depth: float = 0.025, vessel_radius: float = 0.004,
+                 angle_deg: float = 20.0, peak_velocity: float = 0.35,
+                 z_min: float = 0.005, z_max: float = 0.045,
+                 seed: int = 7, with_tissue: bool = True) -> Phantom:
+    """A vessel of laminar (parabolic) flow at `angle_deg` to the probe face,
+    inside stationary speckle whose scatterers are cleared from the lumen
+    (reference: python/usx/phantoms.py:76-97). Only the beam-aligned velocity
+    component, peak_velocity * sin(angle), is visible to Doppler."""
+    med = medium or Medium()
+    ph = Phantom()
+    if with_tissue:
+        half = 0.5 * probe.aperture_width() * 1.15
+        ph = _speckle(probe, med, half, z_min, z_max, 2e-4, seed, 10.0)
+        ph.scale_amplitude_in_circle(Vec3(0.0, 0.0, vessel_depth), vessel_radius, 0.0)
+    ph.add_flow_tube(Vec3(0.0, 0.0, vessel_depth), vessel_radius, 0.06,
+                     float(np.deg2rad(angle_deg)), peak_velocity, 20000, seed + 1, 0.03)
+    return ph
+
+
+def elastography_phantom(probe: Probe, medium: Medium | None = None,
+                         inclusion=(0.0, 0.030, 0.005), stiffness_ratio: float = 4.0,
+                         strain: float = 0.0, z_min: float = 0.010, z_max: float = 0.050,
+                         seed: int = 13) -> Phantom:
+    """Soft speckle with a circular stiff inclusion (x, z, radius), compressed
+    by the bulk axial `strain` from z_min down (0: the reference state; e.g.
+    -0.01 for 1 % compression). The local strain inside the inclusion is
+    strain / stiffness_ratio; each scatterer's displacement integrates the
+    local strain along its axial line, so displacement is continuous across
+    the boundary (reference: python/usx/phantoms.py:100-143)."""
+    med = medium or Medium()
+    half = 0.5 * probe.aperture_width() * 1.15
+    ph = _speckle(probe, med, half, z_min, z_max, 2e-4, seed, 12.0)
+    if strain == 0.0:
+        return ph
+    cx, cz, r = inclusion
+    pos = np.asarray(ph.positions, dtype=np.float64)
+    x, z = pos[:, 0], pos[:, 2]
+    # depth travelled inside the inclusion on the way down to z
+    half_chord = np.sqrt(np.clip(r * r - (x - cx) ** 2, 0.0, None))
+    inside = np.clip(z - (cz - half_chord), 0.0, 2.0 * half_chord)
+    soft = float(strain)
+    stiff = soft / float(stiffness_ratio)
+    moved = pos.copy()
+    moved[:, 2] = z + soft * (z - z_min) + (stiff - soft) * inside
+    out = ph.copy()
+    out.positions = moved.astype(np.float32)
+    out.amplitudes = np.asarray(ph.amplitudes, dtype=np.float32)
+    return out
