@@ -227,7 +227,7 @@ usx_status usx_process_display(usx_ctx* ctx, const usx_demod_desc* d, const usx_
  * axis1 fastest within a frame). `reject` is the wall filter as the row-major
  * n_frames x n_frames projector (I - P) that removes the least-squares fit of
  * a low-order polynomial in slow time, built on the host exactly as
- * polynomial_reject_matrix (doppler.cpp:13-46) builds it; NULL means no
+ * polynomial_reject_matrix (doppler.cpp:15-49) builds it; NULL means no
  * filter (WallFilter::None). */
 typedef struct {
   int n_frames;            /* >= 1 */
@@ -235,7 +235,7 @@ typedef struct {
   const float* reject;     /* host, n_frames * n_frames, or NULL */
 } usx_ensemble_desc;
 
-/* apply_wall_filter (doppler.cpp:48-71): out = reject applied along slow
+/* apply_wall_filter (doppler.cpp:51-75): out = reject applied along slow
  * time at every pixel. ensemble, out: n_frames * n_axis0 * n_axis1 complex. */
 usx_status usx_wall_filter(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c32* ensemble,
                            usx_c32* out, int memory);
@@ -245,7 +245,7 @@ usx_status usx_wall_filter(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c
 usx_status usx_power_doppler(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c32* ensemble,
                              float* power, int memory);
 
-/* DopplerConfig (doppler.hpp:40-79) and the frame facts color_flow reads. */
+/* DopplerConfig (doppler.hpp:41-79) and the frame facts color_flow reads. */
 typedef struct {
   float prf, speed_of_sound;
   float demodulation_frequency;  /* of the ensemble's first frame */
@@ -256,7 +256,7 @@ typedef struct {
   float power_factor;            /* 10^(power_threshold_db / 10), as std::pow(10.f, db / 10.f) */
 } usx_color_flow_desc;
 
-/* color_flow (doppler.cpp:73-176) after the wall filter of e: per pixel the
+/* color_flow (doppler.cpp:77-176) after the wall filter of e: per pixel the
  * axial velocity (m/s, + away from the probe), the post-filter power, the
  * normalised spectral width and the valid flag (1: flow), each
  * n_axis0 * n_axis1. Needs n_frames >= 2. */
@@ -272,13 +272,13 @@ typedef struct {
   int kernel_axial, kernel_lateral, max_lag;  /* as passed; clamped as the reference does */
 } usx_displacement_desc;
 
-/* estimate_axial_displacement (doppler.cpp:188-269): axial displacement (m)
+/* estimate_axial_displacement (doppler.cpp:188-273): axial displacement (m)
  * and |normalised correlation| per pixel from two frames of one grid. */
 usx_status usx_axial_displacement(usx_ctx* ctx, const usx_displacement_desc* d,
                                   const usx_c32* before, const usx_c32* after, float* axial,
                                   float* correlation, int memory);
 
-/* axial_strain (doppler.cpp:271-303): least-squares slope of `axial` against
+/* axial_strain (doppler.cpp:275-307): least-squares slope of `axial` against
  * depth over fit_length / 2 (at least 1) depths either side. */
 usx_status usx_axial_strain(usx_ctx* ctx, int n_axis0, int n_axis1, float dz, int fit_length,
                             const float* axial, float* strain, int memory);

@@ -212,7 +212,7 @@ class OcDoppler(C.Structure):
 
 @dataclass
 class Doppler:
-    """DopplerConfig (reference doppler.hpp:40-79); wall_filter 0 none,
+    """DopplerConfig (reference doppler.hpp:41-79); wall_filter 0 none,
     1 polynomial regression, 2 mean subtraction."""
     wall_filter: int = 1
     polynomial_order: int = 1

@@ -165,21 +165,21 @@ typedef struct {
   int wall_filter, polynomial_order, loupas, kernel_axial, kernel_lateral;
   float power_threshold_db, coherence_threshold;
 } oc_doppler;
-/* apply_wall_filter (doppler.cpp:48-71): out n_frames * n0*n1 complex. */
+/* apply_wall_filter (doppler.cpp:51-75): out n_frames * n0*n1 complex. */
 int oc_wall_filter(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
                    float* out);
 /* power_doppler (doppler.cpp:178-186): power_out n0*n1. */
 int oc_power_doppler(const oc_grid* g, int n_frames, const float* ensemble, int type, int order,
                      float* power_out);
-/* color_flow (doppler.cpp:73-176); fdem is the frames' demodulation frequency. */
+/* color_flow (doppler.cpp:77-176); fdem is the frames' demodulation frequency. */
 int oc_color_flow(const oc_grid* g, int n_frames, const float* ensemble, float fdem, float prf,
                   float c, const oc_doppler* d, float* velocity, float* power, float* variance,
                   uint8_t* valid, float* nyquist);
-/* estimate_axial_displacement (doppler.cpp:188-269). */
+/* estimate_axial_displacement (doppler.cpp:188-273). */
 int oc_axial_displacement(const oc_grid* g, const float* before, const float* after, float fdem,
                           float c, int kernel_axial, int kernel_lateral, int max_lag, float* axial,
                           float* correlation);
-/* axial_strain (doppler.cpp:271-303). */
+/* axial_strain (doppler.cpp:275-307). */
 int oc_axial_strain(const oc_grid* g, const float* axial, int fit_length, float* strain);
 
 #ifdef __cplusplus

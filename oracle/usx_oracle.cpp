@@ -16,7 +16,7 @@
 //   coherence factor, compounding)  core/src/beamformer.cpp:15-214, 297-549
 //   process_frame                   core/src/pipeline.cpp:25-46
 //   wall filter, colour flow, power Doppler, axial displacement and strain
-//                                   core/src/doppler.cpp:13-303
+//                                   core/src/doppler.cpp:15-307
 //
 // Two numeric modes:
 //   FP32 exact order  -- every float expression is evaluated in the
@@ -730,10 +730,10 @@ std::vector<float> display_envelope(const Grid& G, const cf32* frame, float fdem
   return env;
 }
 
-// ---- Doppler and speckle tracking (doppler.cpp:13-303) ----------------------
+// ---- Doppler and speckle tracking (doppler.cpp:15-307) ----------------------
 
 // The (I - P) slow-time projector of the polynomial wall filter
-// (doppler.cpp:13-46): orthonormalised monomials, subtracted outer products.
+// (doppler.cpp:15-49): orthonormalised monomials, subtracted outer products.
 std::vector<float> reject_matrix(int n, int order) {
   order = std::clamp(order, 0, n - 1);
   std::vector<std::vector<float>> basis;
@@ -764,7 +764,7 @@ std::vector<float> reject_matrix(int n, int order) {
   return M;
 }
 
-// Filtered ensemble (doppler.cpp:48-71); type 0 copies.
+// Filtered ensemble (doppler.cpp:51-75); type 0 copies.
 std::vector<cf32> wall_filtered(const float* ens, int n, std::size_t n_pix, int type, int order) {
   const cf32* z = reinterpret_cast<const cf32*>(ens);
   std::vector<cf32> out(z, z + n_pix * static_cast<std::size_t>(n));
@@ -985,7 +985,7 @@ int oc_power_doppler(const oc_grid* g, int n_frames, const float* ensemble, int 
   });
 }
 
-// color_flow (doppler.cpp:73-176): per pixel the window (lateral outer, axial
+// color_flow (doppler.cpp:77-176): per pixel the window (lateral outer, axial
 // inner) and, inside, the ensemble, in one running sum each.
 int oc_color_flow(const oc_grid* g, int n_frames, const float* ensemble, float fdem, float prf,
                   float c, const oc_doppler* d, float* velocity, float* power, float* variance,
@@ -1045,7 +1045,7 @@ int oc_color_flow(const oc_grid* g, int n_frames, const float* ensemble, float f
   });
 }
 
-// estimate_axial_displacement (doppler.cpp:188-269).
+// estimate_axial_displacement (doppler.cpp:188-273).
 int oc_axial_displacement(const oc_grid* g, const float* before, const float* after, float fdem,
                           float c, int kernel_axial, int kernel_lateral, int max_lag, float* axial,
                           float* correlation) {
@@ -1107,7 +1107,7 @@ int oc_axial_displacement(const oc_grid* g, const float* before, const float* af
   });
 }
 
-// axial_strain (doppler.cpp:271-303): least-squares slope over the window.
+// axial_strain (doppler.cpp:275-307): least-squares slope over the window.
 int oc_axial_strain(const oc_grid* g, const float* axial, int fit_length, float* strain) {
   return guarded([&] {
     const Grid G = build_grid(*g);

@@ -2,7 +2,7 @@
 // power Doppler across a slow-time ensemble, and axial speckle tracking and
 // strain between two frames.
 //
-// Interface of the reference's core/include/usx/doppler.hpp:27-141 (the same
+// Interface of the reference's core/include/usx/doppler.hpp:27-142 (the same
 // enum, structs, defaults and free functions). The per-pixel work runs on the
 // GPU through the C ABI (usx_wall_filter, usx_color_flow, usx_power_doppler,
 // usx_axial_displacement, usx_axial_strain); the host builds the wall-filter
@@ -73,7 +73,7 @@ std::vector<real> axial_strain(const DisplacementMap& disp, int fit_length = 15)
 
 // The (I - P) projector of the polynomial wall filter, row-major n x n: P
 // projects onto polynomials of degree <= order in slow time (no reference
-// public counterpart; doppler.cpp:13-46 builds the same matrix).
+// public counterpart; doppler.cpp:15-49 builds the same matrix).
 std::vector<real> polynomial_reject_matrix(int n, int order);
 
 }  // namespace usx

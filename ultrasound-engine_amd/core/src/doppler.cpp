@@ -1,7 +1,7 @@
 // doppler.cpp -- colour / power Doppler and axial speckle tracking on the GPU.
-// Reference: core/src/doppler.cpp (polynomial_reject_matrix 13-46,
-// apply_wall_filter 48-71, color_flow 73-176, power_doppler 178-186,
-// estimate_axial_displacement 188-269, axial_strain 271-303). The host
+// Reference: core/src/doppler.cpp (polynomial_reject_matrix 15-49,
+// apply_wall_filter 51-75, color_flow 77-176, power_doppler 178-186,
+// estimate_axial_displacement 188-273, axial_strain 275-307). The host
 // validates, builds the wall-filter projector and the per-call constants with
 // the reference's float expressions, and packs the frames; the per-pixel work
 // is usx_wall_filter / usx_color_flow / usx_power_doppler /

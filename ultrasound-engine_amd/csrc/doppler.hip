@@ -3,9 +3,9 @@
 // Kasai / Loupas), power Doppler, axial displacement (coarse integer lag +
 // phase refinement) and least-squares axial strain.
 //
-// Reference: core/src/doppler.cpp (apply_wall_filter 48-71, color_flow
-// 73-176, power_doppler 178-186, estimate_axial_displacement 188-269,
-// axial_strain 271-303).
+// Reference: core/src/doppler.cpp (apply_wall_filter 51-75, color_flow
+// 77-176, power_doppler 178-186, estimate_axial_displacement 188-273,
+// axial_strain 275-307).
 //
 // Layout. An ensemble is n_frames frames of n_pix = n0 * n1 complex pixels,
 // frame-major ([frame][axis0][axis1], axis1 fastest), as the frames lie one
@@ -43,7 +43,7 @@ __device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.
 // The NC-register copy of one pixel's ensemble (NC >= n), filtered by the
 // row-major n x n projector M when M is not null: z'_i = sum_j M[i][j] z_j,
 // j in order, each product (z_j re * m, z_j im * m) added to the running sum
-// (doppler.cpp:60-67).
+// (doppler.cpp:66-73).
 template <int NC>
 __device__ __forceinline__ void load_filtered(const float2* __restrict__ in, size_t n_pix, size_t k,
                                               int n, const float* __restrict__ M, float2 (&z)[NC]) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void power_doppler_kernel(int n, size_t n_pix,
 //   lag[0][k] = sum_e |z_e|^2
 //   lag[1..2][k] = sum_{e < n-1} conj(z_e) z_{e+1}       (slow time)
 //   lag[3..4][k] = sum_e conj(z_e[k]) z_e[k + 1]           (axial; 0 on the last depth)
-// (doppler.cpp:119-130 with the window loop taken out). A wavefront loads and
+// (doppler.cpp:115-133 with the window loop taken out). A wavefront loads and
 // filters 64 consecutive pixels and writes the first 63: the axial neighbour's
 // filtered ensemble comes from the next lane by a shuffle, so every pixel is
 // filtered once. Thread 0 of the grid also clears the power maximum.
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void cf_lag_kernel(int n, int n0, int n1,
 }
 
 // ---------------------------------------------------------------------------
-// Colour flow, pass 2: window sums and the estimates (doppler.cpp:111-161).
+// Colour flow, pass 2: window sums and the estimates (doppler.cpp:115-161).
 // A workgroup owns kCfLines lines x 64 depths; it stages the five lag maps of
 // the tile and its halo (kl lines, ka depths either side; zeros off the
 // grid, which add nothing) in LDS, sums each staged row over the 2 ka + 1
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void cf_estimate_global_kernel(ColorFlowArgs A
   if (R00 == R00) atomicMax(pmax_bits, __float_as_uint(R00));
 }
 
-// Power threshold (doppler.cpp:166-174): below pmax * factor is not flow.
+// Power threshold (doppler.cpp:165-174): below pmax * factor is not flow.
 __global__ __launch_bounds__(256) void cf_threshold_kernel(size_t n_pix, const float* __restrict__ power,
                                                            uint8_t* __restrict__ valid,
                                                            const unsigned int* __restrict__ pmax_bits,
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void cf_threshold_kernel(size_t n_pix, const f
 }
 
 // ---------------------------------------------------------------------------
-// Axial displacement between two frames (doppler.cpp:188-269): the local
+// Axial displacement between two frames (doppler.cpp:188-273): the local
 // centre frequency from the first frame's axial phase gradient, the integer
 // lag with the highest normalised correlation, then the residual phase at
 // that lag. Every sum runs in the reference's order.
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void displacement_kernel(DisplacementArgs A,
 }
 
 // Least-squares slope of the displacement against depth over 2 half + 1
-// depths (doppler.cpp:271-303).
+// depths (doppler.cpp:275-307).
 __global__ __launch_bounds__(256) void strain_kernel(int n0, int n1, float dz, int half,
                                                      const float* __restrict__ axial,
                                                      float* __restrict__ strain) {
