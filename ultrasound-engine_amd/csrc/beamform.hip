@@ -128,6 +128,41 @@ __device__ __forceinline__ float2 sample(const void* data, size_t base, int n, f
   return ok ? r : make_float2(0.0f, 0.0f);
 }
 
+// The same sample from a trace addressed as a buffer resource whose extent is
+// exactly the trace: a load outside [0, n) returns 0 instead of touching
+// memory, so the index needs no clamping into the trace, the second sample is
+// the first's offset + one sample (immediate offset), and the address is a
+// 32-bit lane offset from a scalar base. floor(idx) == float(k) wherever the
+// sample is used (0 <= idx < n - 1 < 2^24), so the fraction is idx - floor(idx).
+template <bool IQ>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t trace_rsrc(const void* data, size_t base, int n) {
+  constexpr int kBytes = IQ ? 8 : 4;
+  char* p = const_cast<char*>(static_cast<const char*>(data)) + base * kBytes;
+  // dword 3 of a raw buffer descriptor on gfx9-family parts (32-bit data, no swizzle)
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, n * kBytes, 0x00020000);
+}
+template <bool IQ>
+__device__ __forceinline__ float2 sample_rsrc(__amdgpu_buffer_rsrc_t t, float nm1, float idx) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const bool ok = idx >= 0.0f && idx < nm1;
+  const float fl = floorf(idx);
+  const int k = ok ? static_cast<int>(fl) : 0;
+  const float f = idx - fl;
+  const float g = 1.0f - f;
+  float2 r;
+  if (IQ) {
+    const u2 a = __builtin_amdgcn_raw_buffer_load_b64(t, k * 8, 0, 0);
+    const u2 b = __builtin_amdgcn_raw_buffer_load_b64(t, k * 8 + 8, 0, 0);
+    r = make_float2(fmaf(__uint_as_float(b.x), f, __uint_as_float(a.x) * g),
+                    fmaf(__uint_as_float(b.y), f, __uint_as_float(a.y) * g));
+  } else {
+    const float a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t, k * 4, 0, 0));
+    const float b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t, k * 4 + 4, 0, 0));
+    r = make_float2(fmaf(b, f, a * g), 0.0f);
+  }
+  return ok ? r : make_float2(0.0f, 0.0f);
+}
+
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
 __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, bool valid, int* lo,
                                                int* hi) {
@@ -192,11 +227,12 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
                                                 const TxSlot (&sl)[NE], int nsl, int ch, float a,
                                                 int ev_first) {
   const float trx = rx_time(A.ex[ch], p, A.c);
+  const float nm1 = static_cast<float>(A.n_s - 1);
   float2 acc = make_float2(0.0f, 0.0f);
   // slots in groups of four: one uniform branch per group, and inside it all
   // four loads issued together (a slot past nsl reads slot 0's trace and is
   // not added)
-  constexpr int G = NE < 4 ? NE : 4;
+  constexpr int G = NE == 12 ? 6 : (NE < 4 ? NE : 4);
 #pragma unroll
   for (int s0 = 0; s0 < NE; s0 += G) {
     if (s0 < nsl) {
@@ -208,7 +244,11 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
         const float idx = (sl[on ? s : 0].tau + trx - A.t0) * A.fs;
         const int ev = SEL ? sl[on ? s : 0].ev : ev_first + (on ? s : 0);
         const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
-        v[q] = sample<IQ>(data, base, A.n_s, idx);
+        if (SEL) {
+          v[q] = sample<IQ>(data, base, A.n_s, idx);  // per-lane event: per-lane trace
+        } else {
+          v[q] = sample_rsrc<IQ>(trace_rsrc<IQ>(data, base, A.n_s), nm1, idx);
+        }
       }
 #pragma unroll
       for (int q = 0; q < G; ++q) {
@@ -516,6 +556,8 @@ static void launch_fused_iq(const BfArgs& A, const void* data, float2* frame, in
     launch_fused_ne<4, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
   else if (n_slots <= 8)
     launch_fused_ne<8, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else if (n_slots <= 12)
+    launch_fused_ne<12, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
   else
     launch_fused_ne<16, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
 }
@@ -546,6 +588,8 @@ static void launch_channels_iq(const BfArgs& A, const void* data, float2* Y, uin
     launch_channels_ne<4, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
   else if (n_slots <= 8)
     launch_channels_ne<8, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else if (n_slots <= 12)
+    launch_channels_ne<12, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
   else
     launch_channels_ne<16, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
 }
