@@ -212,7 +212,36 @@ def doppler_leg(usx, torch, frame, grid, steps, warmup, n_frames=16, prf=5000.0)
     pd_bytes = n_frames * n_pix * 8 + n_pix * 4
     out["power_doppler_hbm_frac"] = pd_bytes / (out["power_doppler_kernel_ms"] * 1e-3) / PEAK_HBM
     out["valid_fraction"] = float(ok.float().mean().item())
-    return out
+    return out, ens, dc
+
+
+def cpu_das_and_doppler(usx, probe, med, acq, txs, raw, grid, ens, dc, prf=5000.0):
+    """The reference's CPU DAS process_frame on the full frame (its OpenMP
+    threads) and its colour flow on the same 16-frame ensemble as the GPU leg
+    (single-threaded in the reference), each timed once after a warm-up of
+    the smaller one."""
+    from oracle import oracle as O
+
+    flavour = "reference" if O.available("reference") else "port"
+    orc = O.load(flavour)
+    s = O.Setup.of(probe, med, acq)
+    otx = [O.Tx.of(t) for t in txs]
+    half = 0.5 * probe.aperture_width()
+    gs = O.GridSpec(0, -half, half, 256, 0.005, 0.050, 640)
+    bf = O.Bf(f_number=1.5, combiner=0)
+    rf = np.asarray(raw.rf)
+    t0 = time.perf_counter()
+    orc.process_frame(s, otx, rf, gs, O.Demod(decimation=4), bf)
+    das_s = time.perf_counter() - t0
+    d = O.Doppler.of(dc)
+    t0 = time.perf_counter()
+    orc.color_flow(gs, ens, 5.0e6, prf, 1540.0, d)
+    cf_s = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return ({"value": 1.0 / das_s, "unit": "frames/s", "cores": threads, "kind": flavour,
+             "sample": "DAS process_frame, full 256 x 640 frame"},
+            {"ms": cf_s * 1e3, "cores": 1, "kind": flavour,
+             "sample": f"color_flow on the {ens.shape[0]}-frame 256 x 640 ensemble"})
 
 
 def time_host(usx, raw, grid, pc, steps):
@@ -397,7 +426,8 @@ def main():
     extras["das_hbm_frac"] = das_bytes / (das_total_ms * 1e-3) / PEAK_HBM
 
     log("Doppler (16-frame ensemble, device-resident) ...")
-    extras["doppler"] = doppler_leg(usx, torch, das_frame, grid, max(a.steps, 50), a.warmup)
+    extras["doppler"], d_ens, d_cfg = doppler_leg(usx, torch, das_frame, grid, max(a.steps, 50),
+                                                  a.warmup)
     log(f"  {extras['doppler']}")
 
     if not a.skip_stream:
@@ -409,6 +439,13 @@ def main():
         cpu = cpu_baseline(usx, probe, med, acq, txs, raw)
         log(f"  {cpu}")
         extras["mv_speedup_vs_cpu"] = (1e3 / mv_ms) / cpu["value"]
+        log("CPU DAS frame and colour flow (reference) ...")
+        cpu_das, cpu_cf = cpu_das_and_doppler(usx, probe, med, acq, txs, raw, grid, d_ens, d_cfg)
+        extras["cpu_das"] = cpu_das
+        extras["das_speedup_vs_cpu"] = extras["das_fps"] / cpu_das["value"]
+        extras["doppler"]["cpu_color_flow"] = cpu_cf
+        extras["doppler"]["color_flow_speedup_vs_cpu"] = cpu_cf["ms"] / extras["doppler"]["color_flow_ms"]
+        log(f"  DAS {cpu_das}, colour flow {cpu_cf}")
 
     line = {
         "metric": METRIC, "value": 1e3 / mv_ms, "unit": "frames/s", "n_gpus": 1,
