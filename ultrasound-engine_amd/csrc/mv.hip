@@ -764,6 +764,54 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
 // * No back substitution: with w = R^-1 1, sum w = 1^H R^-1 1 = |y1|^2 and
 //   w^H avg = y1^H y2 (y1 = L^-1 1, y2 = L^-1 avg), so
 //   out = (y1^H y2) / |y1|^2 * n.
+// Trailing update of the Cholesky block [j0, j0 + 8) on MFMA, for every
+// pixel group gg of the wavefront: S(r, j0 + c) = A(r, j0 + c) -
+// sum_{k < j0} L(r, k) conj(L(j0 + c, k)), written back into the packed lower
+// triangle in place (rows r >= j0 + c of the columns j0 + c < L of the group).
+// Complex products are expanded to real: with a = L(r, k), b = L(j0 + c, k),
+//   Re S -= [a_r a_i] . [b_r  b_i],   Im S -= [a_r a_i] . [-b_i  b_r],
+// a (rows x 2 j0) . (2 j0 x 16) real product on v_mfma_f32_16x16x4_f32: 16
+// rows per tile, 4 real k (2 complex columns) per step, accumulated onto the
+// triangle's entries with B negated. Lane l supplies A[row l % 16][k-part
+// l / 16] and B[k-part l / 16][column l % 16] and holds D[4 (l / 16) + i][l % 16].
+// Tiles and steps have wavefront-uniform bounds; rows and columns a group
+// does not own produce values that are never stored. tri_off: byte offset of
+// the triangle in a group's LDS, group_bytes: LDS per group.
+template <int P, int G>
+__device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t group_bytes,
+                                                  size_t tri_off, int lane, int j0, int Lw,
+                                                  int L) {
+  constexpr int LP = 64 / G;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int c = lr >> 1, part = lr & 1;
+  const int kp = lq & 1;
+  const int t0 = j0 >> 4, t1 = (Lw + 15) >> 4;
+  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    float* tri = reinterpret_cast<float*>(smem + gg * group_bytes + tri_off);
+    const int Lg = G == 1 ? L : __shfl(L, gg * LP, 64);
+    for (int mt = t0; mt < t1; ++mt) {
+      f4 acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = tri[2 * at(16 * mt + 4 * lq + i, j0 + c) + part];
+      for (int st = 0; st < (j0 >> 1); ++st) {
+        const int k = 2 * st + (lq >> 1);
+        const float a = tri[2 * at(16 * mt + lr, k) + kp];
+        const float br = tri[2 * at(j0 + c, k)], bi = tri[2 * at(j0 + c, k) + 1];
+        // -B[2k + kp][2c + part]
+        const float nb = part == 0 ? -(kp ? bi : br) : (kp ? -br : bi);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, nb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * mt + 4 * lq + i;
+        if (r >= j0 + c && j0 + c < Lg) tri[2 * at(r, j0 + c) + part] = acc[i];
+      }
+    }
+  }
+}
+
 template <int P, int NR, int RS, int G>
 __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
                                                     const uint32_t* __restrict__ mask,
@@ -783,7 +831,9 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   // packed by columns -- (i >= k, k) at k P - k (k + 1) / 2 + i, so a
   // column's rows are contiguous (lane-consecutive reads and stores)
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  unsigned char* mine = smem_raw + g * (ints + sizeof(float2) * (NR * RS + TRI));
+  const size_t group_bytes = ints + sizeof(float2) * (NR * RS + TRI);
+  const size_t tri_off = ints + sizeof(float2) * (NR * RS);
+  unsigned char* mine = smem_raw + g * group_bytes;
   int* act = reinterpret_cast<int*>(mine);
   float2* rows = reinterpret_cast<float2*>(mine + ints);
   float2* A = rows + NR * RS;
@@ -897,18 +947,26 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     __syncthreads();
 
     // ---- Cholesky, left-looking in blocks of CB columns, with both forward
-    // substitutions. For the block [j0, j0 + CB) lane i first forms
-    //   s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k))
-    // (one read of its own row per k feeds CB accumulators), then finishes the
-    // block column by column with the in-block terms k = j0 .. j0+c-1 (taken
-    // by readlane from the registers of lane j0+c), so every sum runs over k
-    // in increasing order as in the reference (beamformer.cpp:48-63).
+    // substitutions. For the block [j0, j0 + CB) the terms of the finished
+    // columns, s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k)), are formed
+    // on MFMA in the triangle (block_update_mfma), or by lane i for its own
+    // row (one read of its row per k feeds CB accumulators, k increasing);
+    // lane i then finishes its row of the block column by column with the
+    // in-block terms k = j0 .. j0+c-1 (taken by readlane from the registers of
+    // lane j0+c), in increasing k as the reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
+    // the finished columns' terms on MFMA (measured faster for one and four
+    // pixels per wavefront) or on the VALU (two pixels per wavefront)
+    constexpr bool kMfmaUpdate = G != 2;
     float b1r = sub < L ? 1.0f : 0.0f, b1i = 0.0f;
     bool ok = true;  // this group's pivots so far are positive
     for (int j0 = 0; j0 < Lw; j0 += CB) {
       // Lanes outside [j0, L) keep zero partial sums, so their factor entries
       // are 0 and the substitution updates below leave their b unchanged.
+      if (kMfmaUpdate && j0 > 0) {
+        block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, Lw, L);
+        __syncthreads();
+      }
       float sr[CB], si[CB], lr[CB], li[CB];
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
@@ -924,17 +982,19 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
             si[c] = v.y;
           }
         }
+        if constexpr (!kMfmaUpdate) {
 #pragma unroll 2
-        for (int k = 0; k < j0; ++k) {
-          // column k: this lane's row, and the block's rows j0 .. j0+CB-1 (contiguous;
-          // rows >= L hold stale values whose sums are never used, and j0 + CB <= P)
-          const float2* colk = A + at(0, k);
-          const float2 a = colk[sub];
+          for (int k = 0; k < j0; ++k) {
+            // column k: this lane's row, and the block's rows j0 .. j0+CB-1 (contiguous;
+            // rows >= L hold stale values whose sums are never used, and j0 + CB <= P)
+            const float2* colk = A + at(0, k);
+            const float2 a = colk[sub];
 #pragma unroll
-          for (int c = 0; c < CB; ++c) {
-            const float2 b = colk[j0 + c];  // -= a conj(b)
-            sr[c] = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr[c]));
-            si[c] = fmaf(-a.y, b.x, fmaf(a.x, b.y, si[c]));
+            for (int c = 0; c < CB; ++c) {
+              const float2 b = colk[j0 + c];  // -= a conj(b)
+              sr[c] = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr[c]));
+              si[c] = fmaf(-a.y, b.x, fmaf(a.x, b.y, si[c]));
+            }
           }
         }
       }
