@@ -236,7 +236,8 @@ typedef struct {
 } usx_ensemble_desc;
 
 /* apply_wall_filter (doppler.cpp:51-75): out = reject applied along slow
- * time at every pixel. ensemble, out: n_frames * n_axis0 * n_axis1 complex. */
+ * time at every pixel. ensemble, out: n_frames * n_axis0 * n_axis1 complex,
+ * separate buffers (they must not overlap). */
 usx_status usx_wall_filter(usx_ctx* ctx, const usx_ensemble_desc* e, const usx_c32* ensemble,
                            usx_c32* out, int memory);
 

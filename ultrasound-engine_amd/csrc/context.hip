@@ -1561,6 +1561,8 @@ usx_status usx_wall_filter(usx_ctx* c, const usx_ensemble_desc* e, const usx_c32
   return guarded(c, [&] {
     const size_t n_pix = ensemble_pixels(c, e, 1, "apply_wall_filter");
     if (!ensemble || !out) fail(c, USX_ERR_INVALID, "apply_wall_filter: null data pointer");
+    if (static_cast<const void*>(ensemble) == static_cast<const void*>(out))
+      fail(c, USX_ERR_INVALID, "apply_wall_filter: the output must not overwrite the ensemble");
     const size_t n = n_pix * e->n_frames;
     const float* M = reject_on_device(c, e);
     const float2* d_in = ensemble_on_device(c, e, ensemble, memory, c->dop_in);
