@@ -193,7 +193,8 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
       fail(c, USX_ERR_INVALID,
            "MV: subarray length up to %d with %d channels exceeds the supported %d",
            std::min(b->mv_subarray, b->n_channels / 2), b->n_channels, usxd::kMvMaxP);
-    const size_t lds = usxd::mv_smem_bytes(b->n_channels, b->mv_range_average, b->mv_subarray, pmax);
+    const size_t lds =
+        usxd::mv_smem_bytes(b->n_channels, b->mv_range_average, b->mv_subarray, pmax, b->mv_refine != 0);
     if (lds > 160 * 1024)
       fail(c, USX_ERR_INVALID,
            "MV: %d channels with range average %d need %zu B of LDS per pixel (limit 163840)",

@@ -68,7 +68,9 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
                int* counts, int* lists, hipStream_t st);
 // Largest covariance size bucket (L rounded up to 16) a configuration can produce.
 int mv_max_bucket_p(int n_ch, int subarray);
-size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P);
+// Dynamic LDS per wavefront of the general MV kernel for bucket P (refine:
+// the full matrix, else its packed lower triangle).
+size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P, bool refine);
 constexpr int kMvMaxP = 96;
 
 // apod.hip: learned apodization MLP (256 -> 3 x 256 GELU -> 256, bf16 MFMA).

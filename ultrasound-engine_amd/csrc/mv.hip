@@ -31,17 +31,23 @@
 // per pixel, LDS sized for the bucket:
 //   1. active list; centre and +-range_avg neighbour vectors staged in LDS
 //      (neighbours clipped at the grid edge, beamformer.cpp:159-161);
-//   2. covariance R = sum over (depth, subarray start) of x x^H as a dense
-//      rank-k update on MFMA, complex expanded to real:
+//   2. covariance R = sum over (depth, subarray start) of x x^H: with FP32
+//      inputs by the shift recursion above, lane m walking sub-diagonals
+//      m + 64 q; with BF16/FP16 inputs as a dense rank-k update on MFMA,
+//      complex expanded to real:
 //        Re R = [xr xi][xr xi]^T,  Im R = [xi -xr][xr xi]^T,
 //      v_mfma_f32_16x16x4_f32, two snapshots per instruction (k = re/im of
 //      snapshot t, re/im of snapshot t+1), L padded to P with zero rows,
 //      upper-triangle 16x16 blocks only;
+//      R is kept in LDS as the full matrix when refinement is on, else as
+//      its lower triangle packed by rows (half the LDS, so twice the pixels
+//      resident per CU);
 //   3. R /= n_snap; diagonal loading max(delta tr(R)/L, 1e-20) with the trace
 //      summed in FP64;
-//   4. in-place FP32 Cholesky in LDS, left-looking by columns (the strictly
-//      upper triangle keeps R for the residual); a non-positive pivot -> DAS
-//      sum (beamformer.cpp:199);
+//   4. in-place FP32 Cholesky in LDS, left-looking in blocks of 8 columns:
+//      the finished columns' terms on MFMA, the in-block terms by the row's
+//      lane (the strictly upper triangle keeps R for the residual); a
+//      non-positive pivot -> DAS sum (beamformer.cpp:199);
 //   5. triangular solves for R w = 1 with the vector in registers (lane i
 //      holds w_i) and the pivot broadcast by readlane: no barriers;
 //   6. optional single refinement step: r = 1 - R w in FP64, R d = r, w += d;
@@ -272,12 +278,15 @@ __device__ __forceinline__ Staged stage_group(const MvArgs& M, const float2* __r
 
 }  // namespace
 
-// Bytes of dynamic LDS for one wavefront of bucket P.
-size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
+// Bytes of dynamic LDS for one wavefront of bucket P: the full matrix with
+// refinement (its upper triangle keeps R for the residual), else the lower
+// triangle packed by rows.
+size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P, bool refine) {
   const int na_max = subarray <= P ? n_ch : std::min(n_ch, 2 * P + 1);
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max +
-                                  static_cast<size_t>(P) * (P + 1) + 2 * static_cast<size_t>(P));
+  const size_t matrix = refine ? static_cast<size_t>(P) * (P + 1) : static_cast<size_t>(P) * (P + 1) / 2;
+  return ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * na_max + matrix +
+                                  2 * static_cast<size_t>(P));
 }
 
 // Row stride of mv_solve_fast's staged vectors (float2) for bucket P, or 0
@@ -375,6 +384,50 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   if (take) lists[static_cast<size_t>(b) * n_pix + slot] = pix;
 }
 
+// Trailing update of the general path's Cholesky block [j0, j0 + 8) on MFMA,
+// in the full matrix (rows of stride P + 1 complex, factor in the lower
+// triangle): S(r, j0 + c) = A(r, j0 + c) - sum_{k < j0} L(r, k) conj(L(j0 + c, k))
+// for rows j0 + c <= r < L, written back in place. Complex products expanded
+// to real: with a = L(r, k), b = L(j0 + c, k),
+//   Re S -= [a_r a_i] . [b_r  b_i],   Im S -= [a_r a_i] . [-b_i  b_r],
+// a (rows x 2 j0) . (2 j0 x 16) real product on v_mfma_f32_16x16x4_f32: 16 rows
+// per tile, 2 complex columns per step, accumulated onto the matrix entries
+// with B negated. Lane l supplies A[row l % 16][k-part l / 16] and
+// B[k-part l / 16][column l % 16] and holds D[4 (l / 16) + i][l % 16]. Rows of
+// a tile at or past L give values never stored. packed: the lower triangle
+// by rows, (i, k) at i (i + 1) / 2 + k; entries read outside it (k > i, from
+// rows never stored) still lie inside the P (P + 1) / 2 allocation for i, k < P.
+template <int P>
+__device__ __forceinline__ void block_update_full(float* A, bool packed, int lane, int j0, int L) {
+  auto base = [packed](int r) { return 2 * (packed ? (r * (r + 1)) >> 1 : r * (P + 1)); };
+  const int lr = lane & 15, lq = lane >> 4;
+  const int c = lr >> 1, part = lr & 1;
+  const int kp = lq & 1;
+  const int col = j0 + c;  // < P: j0 < L <= P, both multiples of 8
+  const int t1 = (L + 15) >> 4;
+  const float* bq = A + base(col);
+  for (int mt = j0 >> 4; mt < t1; ++mt) {
+    f4 acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = A[base(16 * mt + 4 * lq + i) + 2 * col + part];
+    const float* arow = A + base(16 * mt + lr) + kp;
+#pragma unroll 2
+    for (int st = 0; st < (j0 >> 1); ++st) {
+      const int k = 2 * st + (lq >> 1);
+      const float a = arow[2 * k];
+      const float br = bq[2 * k], bi = bq[2 * k + 1];
+      // -B[2k + kp][2c + part]
+      const float nb = part == 0 ? -(kp ? bi : br) : (kp ? -br : bi);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, nb, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * mt + 4 * lq + i;
+      if (r >= col && r < L && col < L) A[base(r) + 2 * col + part] = acc[i];
+    }
+  }
+}
+
 template <int P, int PREC>
 __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restrict__ Y,
                                                const uint32_t* __restrict__ mask,
@@ -392,12 +445,16 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
   const int n_pix = M.n0 * M.n1;
   int* act = reinterpret_cast<int*>(smem_raw);
   float2* rows = reinterpret_cast<float2*>(smem_raw + (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16);
-  // A: P rows of stride P + 1 (conflict-free column access). Strictly upper
-  // triangle: R (kept for the residual); lower triangle + diagonal: the
-  // Cholesky factor. dg[a] = (loaded diagonal of R, 1 / L_aa).
+  // A with refinement: P rows of stride P + 1 (conflict-free column access),
+  // strictly upper triangle R (kept for the residual), lower triangle +
+  // diagonal the Cholesky factor. Without refinement the upper triangle is
+  // never read: the lower triangle alone, packed by rows, halves the LDS per
+  // pixel. dg[a] = (loaded diagonal of R, 1 / L_aa).
   constexpr int S = P + 1;
+  const bool packed = !M.refine;
+  auto ix = [packed](int i, int k) { return packed ? ((i * (i + 1)) >> 1) + k : i * S + k; };
   float2* A = rows + static_cast<size_t>(2 * M.range_avg + 1) * na_max;
-  float2* dg = A + P * S;
+  float2* dg = A + (packed ? P * (P + 1) / 2 : P * S);
   float2* rv = dg + P;  // scratch vector
 
   const int count = counts[BUCKET];
@@ -433,7 +490,63 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       }
     }
     const float inv = unsc2 / static_cast<float>(n_snap);
-    {
+    if constexpr (PREC == 0) {
+      // FP32 inputs: the subarray shift recursion along the sub-diagonals,
+      // lane owns m = lane + 64 q,
+      //   R(m,0) = sum_r sum_{a<ns} x_r[a+m] conj(x_r[a]),
+      //   R(m+t+1,t+1) = R(m+t,t) + sum_r x_r[m+t+ns] conj(x_r[t+ns]) - x_r[m+t] conj(x_r[t]),
+      // O(L n_snap + L^2 nrow) products instead of the rank-k update's
+      // O(L^2 n_snap). Lanes past their diagonal's end read clamped indices
+      // and store nothing.
+      float cr[NS], ci[NS];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        cr[q] = 0.0f;
+        ci[q] = 0.0f;
+      }
+      for (int r = 0; r < nrow; ++r) {
+        const float2* x = rows + r * na;
+#pragma unroll 2
+        for (int a = 0; a < ns; ++a) {
+          const float2 b = x[a];
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const float2 v = x[a + min(lane + 64 * q, L - 1)];
+            cr[q] = fmaf(v.x, b.x, fmaf(v.y, b.y, cr[q]));
+            ci[q] = fmaf(v.y, b.x, fmaf(-v.x, b.y, ci[q]));
+          }
+        }
+      }
+      for (int t = 0;; ++t) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const int m = lane + 64 * q, i = m + t;
+          if (i < L) {
+            const float2 v = make_float2(cr[q] * inv, ci[q] * inv);
+            if (m == 0) {
+              dg[t] = make_float2(v.x, 0.0f);
+            } else {
+              A[ix(i, t)] = v;                                   // factor input, lower
+              if (!packed) A[t * S + i] = make_float2(v.x, -v.y);  // R, upper
+            }
+          }
+        }
+        if (t + 1 >= L) break;
+        const float2* x = rows;
+        for (int r = 0; r < nrow; ++r, x += na) {
+          const float2 hk = x[t + ns], lk = x[t];
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const int il = min(lane + 64 * q + t, L - 2);
+            const float2 hi = x[il + ns], lo = x[il];
+            cr[q] = fmaf(hi.x, hk.x, fmaf(hi.y, hk.y, cr[q]));
+            ci[q] = fmaf(hi.y, hk.x, fmaf(-hi.x, hk.y, ci[q]));
+            cr[q] = fmaf(-lo.x, lk.x, fmaf(-lo.y, lk.y, cr[q]));
+            ci[q] = fmaf(-lo.y, lk.x, fmaf(lo.x, lk.y, ci[q]));
+          }
+        }
+      }
+    } else {
       // All upper-triangle block pairs in one pass: per snapshot pair, one
       // operand load per 16-row block, then 2 MFMAs per block pair.
       f4 accR[NPAIRS], accI[NPAIRS];
@@ -495,8 +608,8 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
               if (row == col) {
                 dg[row] = make_float2(v.x, 0.0f);
               } else {
-                A[row * S + col] = v;                       // R, upper
-                A[col * S + row] = make_float2(v.x, -v.y);  // factor input, lower
+                if (!packed) A[row * S + col] = v;            // R, upper
+                A[ix(col, row)] = make_float2(v.x, -v.y);  // factor input, lower
               }
             }
           }
@@ -513,50 +626,91 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
     for (int a = lane; a < L; a += 64) {
       const float d = dg[a].x + eps;
       dg[a] = make_float2(d, 0.0f);
-      A[a * S + a] = make_float2(d, 0.0f);
+      A[ix(a, a)] = make_float2(d, 0.0f);
     }
     __syncthreads();
 
-    // ---- 5. Cholesky, left-looking by columns: lane i owns rows i, i + 64 ----
+    // ---- 5. Cholesky, left-looking in blocks of CB columns: lane i owns rows
+    // i, i + 64 --------------------------------------------------------------
     //   L_ij = (A_ij - sum_{k<j} L_ik conj(L_jk)) / L_jj, L_jj = sqrt(A_jj - sum |L_jk|^2),
-    // the reference's sums (beamformer.cpp:48-63); one barrier per column.
+    // the reference's sums (beamformer.cpp:48-63). For the block [j0, j0 + CB)
+    // the finished columns' terms k < j0 are subtracted on MFMA in place
+    // (block_update_full); lane i then loads its rows of the block and finishes
+    // them column by column with the in-block terms k = j0 .. j - 1, taken by
+    // readlane from the lane that owns row j, k increasing. Two barriers per
+    // block.
+    constexpr int CB = 8;
     bool ok = true;
-    for (int j = 0; j < L; ++j) {
-      float sr[NS], si[NS];
+    for (int j0 = 0; j0 < L && ok; j0 += CB) {
+      if (j0 > 0) {
+        block_update_full<P>(reinterpret_cast<float*>(A), packed, lane, j0, L);
+        __syncthreads();
+      }
+      float sr[NS][CB], si[NS][CB], rjs[CB];
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const int row = lane + 64 * q;
-        sr[q] = 0.0f;
-        si[q] = 0.0f;
-        if (row >= j && row < L) {
-          const float2* ri_ = A + row * S;
-          const float2* rj = A + j * S;
-          float ar = ri_[j].x, ai = ri_[j].y;
-#pragma unroll 4
-          for (int k = 0; k < j; ++k) {
-            const float2 a = ri_[k], b = rj[k];  // -= a * conj(b)
-            ar = fmaf(-a.x, b.x, fmaf(-a.y, b.y, ar));
-            ai = fmaf(-a.y, b.x, fmaf(a.x, b.y, ai));
-          }
-          sr[q] = ar;
-          si[q] = ai;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const int j = j0 + c;
+          float2 v = make_float2(0.0f, 0.0f);
+          if (j < L && row >= j && row < L) v = A[ix(row, j)];
+          sr[q][c] = v.x;
+          si[q][c] = v.y;
         }
       }
-      const float d = lane_bcast(pick<NS>(sr, j >> 6), j & 63);
-      if (!(d > 0.0f)) {
-        ok = false;
-        break;
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        const int j = j0 + c;
+        rjs[c] = 0.0f;
+        if (j >= L || !ok) continue;
+#pragma unroll
+        for (int u = 0; u < c; ++u) {  // in-block terms, k = j0 + u: -= L_ik conj(L_jk)
+          float tr_[NS], ti_[NS];
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            tr_[q] = sr[q][u];
+            ti_[q] = si[q][u];
+          }
+          const float br = lane_bcast(pick<NS>(tr_, j >> 6), j & 63);
+          const float bi = lane_bcast(pick<NS>(ti_, j >> 6), j & 63);
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const float ar = sr[q][u], ai = si[q][u];
+            sr[q][c] = fmaf(-ar, br, fmaf(-ai, bi, sr[q][c]));
+            si[q][c] = fmaf(-ai, br, fmaf(ar, bi, si[q][c]));
+          }
+        }
+        float dq[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) dq[q] = sr[q][c];
+        const float d = lane_bcast(pick<NS>(dq, j >> 6), j & 63);
+        if (!(d > 0.0f)) {
+          ok = false;
+          continue;
+        }
+        const float ljj = sqrtf(d);
+        const float rj = 1.0f / ljj;
+        rjs[c] = rj;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const int row = lane + 64 * q;
+          const bool piv = row == j;
+          sr[q][c] = piv ? ljj : sr[q][c] * rj;  // rows above j are not stored
+          si[q][c] = piv ? 0.0f : si[q][c] * rj;
+        }
       }
-      const float ljj = sqrtf(d);
-      const float rj = 1.0f / ljj;
+      if (!ok) break;
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const int row = lane + 64 * q;
-        if (row == j) {
-          A[j * S + j] = make_float2(ljj, 0.0f);
-          dg[j].y = rj;
-        } else if (row > j && row < L) {
-          A[row * S + j] = make_float2(sr[q] * rj, si[q] * rj);
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const int j = j0 + c;
+          if (j < L && row >= j && row < L) {
+            A[ix(row, j)] = make_float2(sr[q][c], si[q][c]);
+            if (row == j) dg[j].y = rjs[c];
+          }
         }
       }
       __syncthreads();
@@ -586,7 +740,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
               br[q] = yr;
               bi[q] = yi;
             } else if (row > k && row < L) {
-              const float2 l = A[row * S + k];
+              const float2 l = A[ix(row, k)];
               br[q] = fmaf(-l.x, yr, fmaf(l.y, yi, br[q]));
               bi[q] = fmaf(-l.x, yi, fmaf(-l.y, yr, bi[q]));
             }
@@ -603,7 +757,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
               br[q] = xr;
               bi[q] = xi;
             } else if (row < k) {
-              const float2 l = A[k * S + row];  // -= conj(L[k][row]) * x_k
+              const float2 l = A[ix(k, row)];  // -= conj(L[k][row]) * x_k
               br[q] = fmaf(-l.x, xr, fmaf(-l.y, xi, br[q]));
               bi[q] = fmaf(-l.x, xi, fmaf(l.y, xr, bi[q]));
             }
@@ -1083,7 +1237,7 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
   const bool fast = P <= 64 && M.precision == 0 && !M.refine && M.weights == nullptr &&
                     M.range_avg >= 0 && M.range_avg <= 2 && mv_fast_row_stride(M.n_ch, M.subarray, P) > 0;
   const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
-                           : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P);
+                           : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P, M.refine != 0);
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
   const int n_pix = M.n0 * M.n1;
   // Persistent grid: as many single-wavefront workgroups per CU as can be
