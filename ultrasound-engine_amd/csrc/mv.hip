@@ -16,8 +16,10 @@
 //   1. staging as below, every depth's loads in flight together;
 //   2. covariance by the subarray shift recursion: lane m forms R(m,0)
 //      directly and walks the m-th sub-diagonal,
-//        R(i+1,k+1) = R(i,k) + sum_r x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k]),
-//      O(L n_snap + L^2 nrow) products instead of O(L^2 n_snap);
+//        R(i+1,k+1) = R(i,k) + D(i,k),
+//        D(i,k) = sum_r x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k]),
+//      O(L n_snap + L^2 nrow) products instead of O(L^2 n_snap); all of D at
+//      once on MFMA (a rank-2 nrow product), so a walk step is one add;
 //   3. loading as below;
 //   4. left-looking Cholesky in blocks of 8 columns on the lower triangle
 //      packed by columns, the forward
@@ -189,7 +191,7 @@ __device__ __forceinline__ Staged stage_pixel(const MvArgs& M, const float2* __r
 
 // Lanes of a wavefront split into G groups of LP = 64 / G lanes, one pixel
 // per group. group_bcast: v of lane k of the caller's group (readlanes and
-// selects); group_sum: the sum over the caller's group.
+// selects); group_sum, group_sum_d: the sum over the caller's group.
 template <int G>
 __device__ __forceinline__ float group_bcast(float v, int k, int lane) {
   constexpr int LP = 64 / G;
@@ -202,15 +204,13 @@ __device__ __forceinline__ float group_bcast(float v, int k, int lane) {
   return r;
 }
 template <int G>
-__device__ __forceinline__ double group_bcast_d(double v, int k, int lane) {
-  const long long b = __double_as_longlong(v);
-  const float lo = group_bcast<G>(__int_as_float(static_cast<int>(b & 0xffffffffll)), k, lane);
-  const float hi = group_bcast<G>(__int_as_float(static_cast<int>(b >> 32)), k, lane);
-  return __longlong_as_double((static_cast<long long>(__float_as_int(hi)) << 32) |
-                              static_cast<unsigned int>(__float_as_int(lo)));
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = 32 / G; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 template <int G>
-__device__ __forceinline__ float group_sum(float v) {
+__device__ __forceinline__ double group_sum_d(double v) {
 #pragma unroll
   for (int o = 32 / G; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -966,6 +966,65 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
   }
 }
 
+// Increments of the sub-diagonal walks on MFMA, for every pixel group gg of
+// the wavefront:
+//   D(i, k) = sum_r x_r[i + n_sub] conj(x_r[k + n_sub]) - x_r[i] conj(x_r[k]),  k <= i < L,
+// a rank-2 NR product H H^H - Lo Lo^H of the NR staged rows' high (offset
+// n_sub) and low windows, written into the packed lower triangle at (i, k);
+// R(i+1, k+1) = R(i, k) + D(i, k) then leaves one add per entry to the walks.
+// Complex expanded to real as in block_update_mfma: 16 rows x 8 complex
+// columns per tile, and per MFMA the 4 real terms of one row r (high re, high
+// im, low re, low im -- the low ones with B negated). Indices are clamped to
+// L - 2, so every read lies in the staged entries 0 .. n - 1; row L - 1 gets
+// values the walks overwrite unread. rows_off / tri_off: byte offsets of the
+// rows and of the triangle in a group's LDS, group_bytes: LDS per group.
+template <int P, int NR, int RS, int G>
+__device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t group_bytes,
+                                                     size_t rows_off, size_t tri_off, int lane,
+                                                     int L, int ns) {
+  constexpr int LP = 64 / G;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int c = lr >> 1, part = lr & 1;
+  const int hi = lq >> 1, comp = lq & 1;  // this lane's k-part: window and component
+  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const float2* rows = reinterpret_cast<const float2*>(smem + gg * group_bytes + rows_off);
+    float* tri = reinterpret_cast<float*>(smem + gg * group_bytes + tri_off);
+    const int Lg = G == 1 ? L : __shfl(L, gg * LP, 64);
+    const int nsg = G == 1 ? ns : __shfl(ns, gg * LP, 64);  // every lane takes part
+    const int off = hi ? nsg : 0;
+    const int t1 = (Lg + 15) >> 4, c1 = (Lg + 7) >> 3;
+    for (int mt = 0; mt < t1; ++mt) {
+      const int i = min(16 * mt + lr, Lg - 2) + off;
+      float a[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const float2 v = rows[r * RS + i];
+        a[r] = comp ? v.y : v.x;
+      }
+      const int cbn = min(2 * mt + 2, c1);  // column blocks reaching the tile's rows
+      for (int cb = 0; cb < cbn; ++cb) {
+        const int j = min(8 * cb + c, Lg - 2) + off;
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const float2 w = rows[r * RS + j];
+          // B[k-part][2c + part]: Re: [w_r w_i], Im: [-w_i w_r]; low window negated
+          const float b = part == 0 ? (comp ? w.y : w.x) : (comp ? w.x : -w.y);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], hi ? b : -b, acc, 0, 0, 0);
+        }
+        const int col = 8 * cb + c;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 16 * mt + 4 * lq + e;
+          if (row >= col && row < Lg) tri[2 * at(row, col) + part] = acc[e];
+        }
+      }
+    }
+  }
+}
+
 template <int P, int NR, int RS, int G>
 __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
                                                     const uint32_t* __restrict__ mask,
@@ -1013,73 +1072,108 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     // ones zero) so the loads of a step are all issued before its products;
     // lanes past their diagonal's end (or their pixel's size) read clamped
     // indices and store nothing.
-    double tr = 0.0;  // lane 0 of a group walks the diagonal, trace in FP64
     float cr, ci;
     {
       float ar[2] = {0.0f, 0.0f}, ai[2] = {0.0f, 0.0f};
       const int m = min(sub, L - 1);
-      for (int a = 0; a < nsw; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
-        float2 p[NR][2], q[NR][2];
+      if constexpr (G == 1) {
+        // one pixel: n_sub is uniform, so pairs need no clamping; odd tail after
+        int a = 0;
+        for (; a + 1 < ns; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
+          float2 p[NR][2], q[NR][2];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          const float2* x = rows + r * RS;
+          for (int r = 0; r < NR; ++r) {
+            const float2* x = rows + r * RS;
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int au = min(a + u, ns - 1);
-            p[r][u] = x[au + m];
-            q[r][u] = a + u < ns ? x[au] : make_float2(0.0f, 0.0f);
+            for (int u = 0; u < 2; ++u) {
+              p[r][u] = x[a + u + m];
+              q[r][u] = x[a + u];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const float2 pp = p[r][u], qq = q[r][u];
+              ar[u] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u]));
+              ai[u] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u]));
+            }
           }
         }
+        if (a < ns) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
+          for (int r = 0; r < NR; ++r) {
+            const float2* x = rows + r * RS;
+            const float2 pp = x[a + m], qq = x[a];
+            ar[0] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[0]));
+            ai[0] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[0]));
+          }
+        }
+      } else {  // groups: clamp to each pixel's n_sub
+        for (int a = 0; a < nsw; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
+          float2 p[NR][2], q[NR][2];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const float2 pp = p[r][u], qq = q[r][u];
-            ar[u & 1] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u & 1]));
-            ai[u & 1] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u & 1]));
+          for (int r = 0; r < NR; ++r) {
+            const float2* x = rows + r * RS;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int au = min(a + u, ns - 1);
+              p[r][u] = x[au + m];
+              q[r][u] = a + u < ns ? x[au] : make_float2(0.0f, 0.0f);
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const float2 pp = p[r][u], qq = q[r][u];
+              ar[u & 1] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u & 1]));
+              ai[u & 1] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u & 1]));
+            }
           }
         }
       }
       cr = ar[0] + ar[1];
       ci = ai[0] + ai[1];
     }
-    if (sub < L) A[at(sub, 0)] = make_float2(cr * inv, sub == 0 ? 0.0f : ci * inv);
-    if (sub == 0) tr = static_cast<double>(cr * inv);
-#pragma unroll 2
-    for (int t = 0; t < Lw - 1; ++t) {
-      // R(sub+t+1, t+1) = R(sub+t, t) + sum_r (hi conj(hk) - lo conj(lk))
-      const int il = min(sub + t, L - 2), tk = min(t, L - 2);
-      float2 hi[NR], hk[NR], lo[NR], lk[NR];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const float2* x = rows + r * RS;
-        hi[r] = x[il + ns];
-        hk[r] = x[tk + ns];
-        lo[r] = x[il];
-        lk[r] = x[tk];
-      }
-      float dr = 0.0f, di = 0.0f;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        dr = fmaf(hi[r].x, hk[r].x, fmaf(hi[r].y, hk[r].y, dr));
-        di = fmaf(hi[r].y, hk[r].x, fmaf(-hi[r].x, hk[r].y, di));
-        dr = fmaf(-lo[r].x, lk[r].x, fmaf(-lo[r].y, lk[r].y, dr));
-        di = fmaf(-lo[r].y, lk[r].x, fmaf(lo[r].x, lk[r].y, di));
-      }
-      cr += dr;
-      ci += di;
-      const float vr = cr * inv;
-      if (sub + t + 1 < L) A[at(sub + t + 1, t + 1)] = make_float2(vr, sub == 0 ? 0.0f : ci * inv);
-      if (sub == 0 && t + 1 < L) tr += static_cast<double>(vr);
-    }
-    // ---- diagonal loading max(delta tr / L, 1e-20) ---------------------------
-    tr = group_bcast_d<G>(tr, 0, lane);
-    const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
+    // Walk increments D on MFMA into the triangle, then lane m walks its
+    // sub-diagonal in place: at (m+t, t) it reads D(m+t, t), stores R(m+t, t)
+    // and adds, R(m+t+1, t+1) = R(m+t, t) + D(m+t, t). A lane touches only its
+    // own diagonal, so the reads of 8 steps go out before their stores.
+    walk_increments_mfma<P, NR, RS, G>(smem_raw, group_bytes, ints, tri_off, lane, L, ns);
     __syncthreads();
+    int pos = sub;  // at(sub + t, t); at(i + 1, k + 1) = at(i, k) + P - k
+    for (int t0 = 0; t0 < Lw; t0 += 8) {
+      float2 d[8];
+      int pu[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + u;
+        pu[u] = pos;
+        pos += P - t;
+        d[u] = A[min(pu[u], TRI - 1)];  // past the diagonal's end: read, never stored
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + u;
+        if (sub + t < L) {
+          const float vr = cr * inv;
+          A[pu[u]] = make_float2(vr, sub == 0 ? 0.0f : ci * inv);
+        }
+        cr += d[u].x;
+        ci += d[u].y;
+      }
+    }
+    __syncthreads();  // the main diagonal was written by lane 0 of each group
+    // ---- diagonal loading max(delta tr / L, 1e-20), the trace summed in FP64
+    // (exactly, for FP32 entries of one sign within 2^29 of each other) ------
+    float2* diag = A + at(min(sub, P - 1), min(sub, P - 1));
+    const double tr = group_sum_d<G>(sub < L ? static_cast<double>(diag->x) : 0.0);
+    const float eps = fmaxf(static_cast<float>(static_cast<double>(M.loading) * tr / L), 1e-20f);
     // subarray average of the centre vector: the second right-hand side
     float b2r = 0.0f, b2i = 0.0f;
     if (sub < L) {
-      A[at(sub, sub)].x += eps;
+      diag->x += eps;
       const float2* cen = rows + sg.crow * RS + sub;
       float pr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       for (int s0 = 0; s0 < ns; s0 += 8) {
