@@ -262,44 +262,48 @@ __device__ __forceinline__ void activate(const f4 (&acc)[kMT][NT], u4 (&act)[kKB
 
 }  // namespace
 
-// Weights of the inference kernel move HBM/L2 -> registers -> LDS one stage
-// (kStageTiles output tiles of a layer) at a time, into two LDS buffers used
-// alternately: each thread fetches its kPre 16-byte rows of the next stage
-// into registers before the current stage's MFMAs and stores them into the
-// other buffer after them, so the global loads overlap the compute and one
-// barrier per stage suffices.
+// Weights of the inference kernel move HBM/L2 -> LDS one stage (kStageTiles
+// output tiles of a layer) at a time, into two LDS buffers used alternately,
+// by direct-to-LDS loads (global_load_lds_dwordx4: each lane's 16 bytes land
+// at the wavefront's LDS base + 16 lane, no registers): the next stage's
+// loads are issued before the current stage's MFMAs, into the other buffer,
+// and waited for (vmcnt) before the stage's one barrier.
 constexpr int kStageU4 = kStageTiles * kKB * 64;
-constexpr int kPre = kStageU4 / (64 * kInferWaves);
+constexpr int kPre = kStageU4 / (64 * kInferWaves);  // 16-byte rows per thread and stage
 static_assert(kMT % kStageTiles == 0 && kStageU4 % (64 * kInferWaves) == 0, "stage shape");
 static_assert((kMT / kStageTiles) % 2 == 0, "stages per layer must alternate the buffers evenly");
 
-__device__ __forceinline__ void fetch_stage(const u4* wsrc, int l, int m0, int tid, u4 (&pre)[kPre]) {
+typedef __attribute__((address_space(3))) void* lds_ptr;
+__device__ __forceinline__ void load_stage(const u4* wsrc, int l, int m0, int tid, u4* buf) {
   const u4* src = wsrc + l * (kLayerBf16 / 8) + m0 * kKB * 64;
+  u4* dst = buf + (tid & ~63);  // the wavefront's base, uniform
 #pragma unroll
-  for (int q = 0; q < kPre; ++q) pre[q] = src[tid + q * 64 * kInferWaves];
+  for (int q = 0; q < kPre; ++q)
+    __builtin_amdgcn_global_load_lds(src + tid + q * 64 * kInferWaves,
+                                     (lds_ptr)(dst + q * 64 * kInferWaves), 16, 0, 0);
 }
-__device__ __forceinline__ void store_stage(u4* buf, int tid, const u4 (&pre)[kPre]) {
-#pragma unroll
-  for (int q = 0; q < kPre; ++q) buf[tid + q * 64 * kInferWaves] = pre[q];
-}
+// vmcnt(0), expcnt and lgkmcnt unconstrained: this wavefront's loads into LDS
+// have landed
+__device__ __forceinline__ void wait_stage() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// Layer l from output tile M0 on: fetch the next stage (this layer's next, else
-// layer l + 1's first), run this stage's tiles from its buffer, store the next
-// stage into the other buffer (whose previous stage every wavefront finished
-// before the last barrier), barrier; compile-time recursion over the stages
-// so the accumulator tiles stay register-indexed.
+// Layer l from output tile M0 on: issue the next stage (this layer's next, else
+// layer l + 1's first) into the other buffer (whose previous stage every
+// wavefront finished before the last barrier), run this stage's tiles from
+// its buffer, wait for the loads, barrier; compile-time recursion over the
+// stages so the accumulator tiles stay register-indexed.
 template <int M0>
 __device__ __forceinline__ void stage_and_run(u4* wl, const u4* wsrc, int l, const float* bias,
                                               const u4 (&act)[kKB][kNTi], f4 (&acc)[kMT][kNTi],
                                               int tid, int lane, uint32_t kmask) {
   if constexpr (M0 < kMT) {
     constexpr int B = (M0 / kStageTiles) & 1;
-    u4 pre[kPre];
     const bool next_layer = M0 + kStageTiles >= kMT;
     const bool more = !next_layer || l + 1 < kL;
-    if (more) fetch_stage(wsrc, next_layer ? l + 1 : l, next_layer ? 0 : M0 + kStageTiles, tid, pre);
+    if (more)
+      load_stage(wsrc, next_layer ? l + 1 : l, next_layer ? 0 : M0 + kStageTiles, tid,
+                 wl + (B ^ 1) * kStageU4);
     stage_tiles<M0, kStageTiles, kNTi>(wl + B * kStageU4, bias, act, acc, lane, kmask);
-    if (more) store_stage(wl + (B ^ 1) * kStageU4, tid, pre);
+    wait_stage();
     __syncthreads();
     stage_and_run<M0 + kStageTiles>(wl, wsrc, l, bias, act, acc, tid, lane, kmask);
   }
@@ -331,11 +335,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N,
   const long base = (static_cast<long>(blockIdx.x) * kInferWaves + wv) * (16 * kNTi);
   const int nw = (n_ch + 31) / 32;
   const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
-  {
-    u4 pre[kPre];
-    fetch_stage(wsrc, 0, 0, tid, pre);
-    store_stage(wl, tid, pre);
-  }
+  load_stage(wsrc, 0, 0, tid, wl);  // waited for before the barrier below
   u4 act[kKB][kNTi];
   int na[kNTi];
   uint32_t un[4] = {0u, 0u, 0u, 0u};
@@ -359,6 +359,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N,
     live |= ((u & 0xffffu) ? 1u : 0u) << (2 * q);
     live |= ((u >> 16) ? 1u : 0u) << (2 * q + 1);
   }
+  wait_stage();
   __syncthreads();
   f4 acc[kMT][kNTi];
   for (int l = 0; l < kL; ++l) {
