@@ -15,13 +15,19 @@ MV rate, the learned-apodization rate (trained network shipped in
 usx/data, with its gCNR on the cyst beside MV's and its BF16 MFMA roofline)
 the uneven batched stream (64 frames of seed 1, DAS + MV each, one HIP
 graph) and colour flow / power Doppler / wall filter on a 16-frame ensemble
-of the benchmark grid are reported under "extras".
+of the benchmark grid are reported under "extras", with the MV, DAS and
+learned rates of the same frame captured into a HIP graph and replayed
+(`*_fps_graph`, each checked equal to the eager frame).
+
+Frame rates come from a timed region of eager launches without per-kernel
+events; a second timed region of as many frames, with HIP events recorded
+around every launch, gives the per-kernel times.
 
 roofline: the dominant kernel of the MV step (`mv`, covariance + solve) is
 priced against the FP32 MFMA peak with the reference's FLOP count
 (8 sum n_snap L(L+1)/2 + 8 sum (L^3/6 + L^2) over pixels), divided by the
 kernel's mean duration from HIP events recorded on the library's stream
-over the timed region.
+over the profiled timed region.
 
 cpu_baseline: the reference's own core (oracle/_ref, built from its sources)
 or else this repository's restatement (oracle/), timed on this host's cores
@@ -125,9 +131,32 @@ def mv_executed_flops(probe, grid, f_number=1.5, subarray=64, range_avg=2):
     return float(np.sum(np.where(na >= 4, work, 0.0)))
 
 
-def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False, apod=None):
-    """Device-resident frames/s: RF in HBM before the timed region; the
-    timed region is bracketed by synchronizations on both sides."""
+def _timed(runner, torch, steps, launch):
+    """Wall ms and event ms per frame over `steps` launches, the timed region
+    bracketed by synchronizations on both sides."""
+    torch.cuda.synchronize()
+    runner.synchronize()
+    t0 = time.perf_counter()
+    runner.record(0)
+    for _ in range(steps):
+        launch()
+    runner.record(1)
+    runner.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, runner.elapsed(0, 1) / steps
+
+
+def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False, apod=None, graph=False):
+    """Device-resident frames/s: RF in HBM before the timed regions.
+
+    Returns (ms, ev_ms, kernels, frame, graph_ms): ms / ev_ms per frame over
+    `steps` eager frames (host clock / HIP events); with `profile`, a second
+    timed region of `steps` frames with HIP events recorded around every
+    launch on the library's stream gives the per-kernel ms (kept apart so the
+    events do not slow the throughput figure); with `graph`, the frame
+    captured into a HIP graph and replayed `steps` times (graph_ms, else None).
+    """
     runner = usx.DeviceRunner(grid, pc, raw)
     if apod is not None:
         runner.set_apodization(apod)
@@ -136,25 +165,25 @@ def time_device(usx, torch, raw, grid, pc, steps, warmup, profile=False, apod=No
     torch.cuda.synchronize()
     for _ in range(warmup):
         runner.run(rf.data_ptr(), frame.data_ptr())
-    runner.synchronize()
+    eager = lambda: runner.run(rf.data_ptr(), frame.data_ptr())
+    ms, ev_ms = _timed(runner, torch, steps, eager)
+    kern = {}
     if profile:
         runner.profile(True)
-    torch.cuda.synchronize()
-    runner.synchronize()
-    t0 = time.perf_counter()
-    runner.record(0)
-    for _ in range(steps):
-        runner.run(rf.data_ptr(), frame.data_ptr())
-    runner.record(1)
-    runner.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = runner.elapsed(0, 1)
-    kern = json.loads(runner.profile_report()) if profile else {}
-    if profile:
+        _timed(runner, torch, steps, eager)
+        kern = json.loads(runner.profile_report())
         runner.profile(False)
     out = frame.cpu().numpy()
-    return wall * 1e3 / steps, ev_ms / steps, kern, out
+    graph_ms = None
+    if graph:
+        gframe = torch.empty_like(frame)
+        runner.capture(rf.data_ptr(), gframe.data_ptr())
+        for _ in range(warmup):
+            runner.run_graph()
+        graph_ms, _ = _timed(runner, torch, steps, runner.run_graph)
+        if not torch.equal(gframe, frame):
+            raise RuntimeError("graph replay differs from the eager frame")
+    return ms, ev_ms, kern, out, graph_ms
 
 
 def doppler_leg(usx, torch, frame, grid, steps, warmup, n_frames=16, prf=5000.0):
@@ -373,16 +402,19 @@ def main():
 
     pc_mv = pipeline_config(usx, "MV")
     log("MV device-resident ...")
-    mv_ms, mv_ev_ms, kern, mv_frame = time_device(usx, torch, raw, grid, pc_mv, a.steps,
-                                                  a.warmup, profile=True)
-    log(f"  {mv_ms:.3f} ms/frame (events {mv_ev_ms:.3f}), kernels {kern}")
-    extras = {"mv_fps": 1e3 / mv_ms}
+    mv_ms, mv_ev_ms, kern, mv_frame, mv_graph_ms = time_device(usx, torch, raw, grid, pc_mv,
+                                                               a.steps, a.warmup, profile=True,
+                                                               graph=True)
+    log(f"  {mv_ms:.3f} ms/frame (events {mv_ev_ms:.3f}, graph {mv_graph_ms:.3f}), kernels {kern}")
+    extras = {"mv_fps": 1e3 / mv_ms, "mv_fps_graph": 1e3 / mv_graph_ms}
     das_frame = None
     for name, comb, cf in [("das", "DAS", False), ("das_cf", "DAS", True), ("dmas", "DMAS", False)]:
-        ms, ev_ms, k, fr = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
-                                       max(a.steps, 100), a.warmup, profile=True)
+        ms, ev_ms, k, fr, g_ms = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
+                                             max(a.steps, 100), a.warmup, profile=True,
+                                             graph=name == "das")
         if name == "das":
             das_frame = fr
+            extras["das_fps_graph"] = 1e3 / g_ms
         extras[f"{name}_fps"] = 1e3 / ms
         extras[f"{name}_kernels_ms"] = {n: v[1] / v[0] for n, v in k.items()}
         log(f"  {name}: {ms:.4f} ms/frame, kernels {k}")
@@ -393,10 +425,12 @@ def main():
     apod, _ = L.load_checkpoint(os.path.join(ROOT, "ultrasound-engine_amd", "usx", "data",
                                              "apod_mlp.npz"))
     pc_l = pipeline_config(usx, "Learned")
-    l_ms, _, lk, l_frame = time_device(usx, torch, raw, grid, pc_l, max(a.steps, 50), a.warmup,
-                                       profile=True, apod=apod)
+    l_ms, _, lk, l_frame, l_graph_ms = time_device(usx, torch, raw, grid, pc_l, max(a.steps, 100),
+                                                   a.warmup, profile=True, apod=apod, graph=True)
     extras["learned_fps"] = 1e3 / l_ms
+    extras["learned_fps_graph"] = 1e3 / l_graph_ms
     extras["learned_speedup_vs_mv"] = mv_ms / l_ms
+    extras["learned_speedup_vs_mv_graph"] = mv_graph_ms / l_graph_ms
     extras["learned_kernels_ms"] = {n: v[1] / v[0] for n, v in lk.items()}
     cyst = ((0.0, 0.020), 0.003)
     extras["gcnr_mv"] = usx.metrics.gcnr(np.abs(mv_frame), grid, *cyst)

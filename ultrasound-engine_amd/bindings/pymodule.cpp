@@ -81,6 +81,9 @@ class DeviceRunner {
     dplan_.desc.rotator = reinterpret_cast<const usx_c32*>(dplan_.rotator.data());
     fix_bf();
   }
+  ~DeviceRunner() { drop_graph(); }
+  DeviceRunner(const DeviceRunner&) = delete;
+  DeviceRunner& operator=(const DeviceRunner&) = delete;
   void fix_bf() {
     bplan_.desc.element_x = bplan_.element_x.data();
     bplan_.desc.tx = bplan_.tx.data();
@@ -100,6 +103,26 @@ class DeviceRunner {
                             reinterpret_cast<usx_c32*>(frame_ptr), USX_DEVICE),
             "beamform");
     }
+  }
+  // The frame of run() with the RF and frame pointers bound now, recorded
+  // into a HIP graph (a one-frame usx_plan: one eager pass, then the
+  // capture); run_graph() replays it on the runner's stream.
+  void capture(std::uintptr_t in_ptr, std::uintptr_t frame_ptr) {
+    if (!is_rf_) throw std::invalid_argument("capture: the runner's template must be RF");
+    usx_ctx* c = ctx_.get();
+    drop_graph();
+    const float* rf = reinterpret_cast<const float*>(in_ptr);
+    usx_c32* fr = reinterpret_cast<usx_c32*>(frame_ptr);
+    check(c, usx_plan_create(c, 1, &dplan_.desc, &bplan_.desc, &rf, &fr, &graph_), "plan_create");
+  }
+  void run_graph() {
+    if (!graph_) throw std::logic_error("run_graph: no captured frame (call capture first)");
+    usx_ctx* c = ctx_.get();
+    check(c, usx_plan_launch(c, graph_, 1), "plan_launch");
+  }
+  void drop_graph() {
+    if (graph_) usx_plan_destroy(ctx_.get(), graph_);
+    graph_ = nullptr;
   }
   void synchronize() {
     usx_ctx* c = ctx_.get();
@@ -347,6 +370,7 @@ class DeviceRunner {
   DemodPlan dplan_;
   BeamformPlan bplan_;
   GpuContext ctx_;
+  usx_plan* graph_ = nullptr;
 };
 
 // A fixed sequence of RF frames, each with its own probe, grid, transmits and
@@ -1065,6 +1089,9 @@ PYBIND11_MODULE(_usx, m) {
       .def("run", &DeviceRunner::run, py::arg("input_ptr"), py::arg("frame_ptr"),
            py::call_guard<py::gil_scoped_release>())
       .def("synchronize", &DeviceRunner::synchronize, py::call_guard<py::gil_scoped_release>())
+      .def("capture", &DeviceRunner::capture, py::arg("input_ptr"), py::arg("frame_ptr"),
+           py::call_guard<py::gil_scoped_release>())
+      .def("run_graph", &DeviceRunner::run_graph, py::call_guard<py::gil_scoped_release>())
       .def_property_readonly("stream", &DeviceRunner::stream)
       .def("record", &DeviceRunner::record, py::arg("slot"))
       .def("elapsed", &DeviceRunner::elapsed, py::arg("a"), py::arg("b"))
