@@ -8,6 +8,8 @@ probe, grid and transmit count (usx.FrameSequence, usx.stream).
   configuration (and launch shape), bit for bit.
 * Frames of the stream match the oracle (DAS 1e-5, MV 1e-4).
 * The autotuner's choices change results by float rounding at most.
+* One frame captured by DeviceRunner.capture replays bit for bit (DAS, MV,
+  learned apodization).
 """
 
 from __future__ import annotations
@@ -91,3 +93,34 @@ def test_autotuned_shapes_keep_the_results(usx_mod, small_stream):
         pc.beamformer = b
         plain = np.asarray(usx_mod.ImagingPipeline(grid, pc).process_frame(raw).data)
         assert rel_l2(tuned, plain) < 1e-5, (slots, rel_l2(tuned, plain))
+
+
+@pytest.mark.parametrize("combiner", ["DAS", "MV", "Learned"])
+def test_device_runner_graph_equals_eager(usx_mod, O, combiner):
+    """DeviceRunner.capture_host records one frame into a HIP graph (a
+    one-frame plan on the runner's device copy of the RF); every replay, with
+    other device work between them, equals ImagingPipeline.process_frame of
+    the same data bit for bit."""
+    import os
+    from cases import bf_config, make_case
+    from usx import learned as L
+    case = make_case(usx_mod, O, n_el=128, n_samples=1024, n_tx=3, n_scat=800,
+                     grid=(16, 0.008, 0.030, 48), box=(-0.02, 0.02, 0.005, 0.032))
+    pc = usx_mod.PipelineConfig()
+    pc.demod.decimation = 4
+    pc.beamformer = bf_config(usx_mod, combiner, subarray=16)
+    pipe = usx_mod.ImagingPipeline(case.grid, pc)
+    run = usx_mod.DeviceRunner(case.grid, pc, case.raw)
+    if combiner == "Learned":
+        ckpt = os.path.join(os.path.dirname(usx_mod.__file__), "data", "apod_mlp.npz")
+        params = L.load_checkpoint(ckpt)[0]
+        pipe.set_apodization(params)
+        run.set_apodization(params)
+    ref = np.asarray(pipe.process_frame(case.raw).data)
+    assert np.abs(ref).max() > 0
+    run.capture_host(case.raw)
+    for _ in range(3):
+        pipe.process_frame(case.raw)  # other device work between the replays
+        run.run_graph()
+        run.synchronize()
+        assert np.array_equal(_bits(run.graph_frame()), _bits(ref))

@@ -110,7 +110,8 @@ class DeviceRunner {
   void capture(std::uintptr_t in_ptr, std::uintptr_t frame_ptr) {
     if (!is_rf_) throw std::invalid_argument("capture: the runner's template must be RF");
     usx_ctx* c = ctx_.get();
-    drop_graph();
+    if (graph_) usx_plan_destroy(c, graph_);
+    graph_ = nullptr;
     const float* rf = reinterpret_cast<const float*>(in_ptr);
     usx_c32* fr = reinterpret_cast<usx_c32*>(frame_ptr);
     check(c, usx_plan_create(c, 1, &dplan_.desc, &bplan_.desc, &rf, &fr, &graph_), "plan_create");
@@ -120,9 +121,36 @@ class DeviceRunner {
     usx_ctx* c = ctx_.get();
     check(c, usx_plan_launch(c, graph_, 1), "plan_launch");
   }
+  // capture() on device copies of host channel data of the template's shape,
+  // which the runner keeps with the graph; graph_frame() downloads the
+  // frame the replays write.
+  void capture_host(const ChannelData& data) {
+    if (!is_rf_) throw std::invalid_argument("capture_host: the runner's template must be RF");
+    if (data.rf.size() != static_cast<std::size_t>(dplan_.desc.n_traces) * dplan_.desc.n_in)
+      throw std::invalid_argument("channel data does not match the runner's template");
+    usx_ctx* c = ctx_.get();
+    drop_graph();
+    check(c, usx_buffer_alloc(c, data.rf.size() * sizeof(float), &graph_rf_), "buffer_alloc");
+    check(c, usx_buffer_alloc(c, sizeof(usx_c32) * frame_pixels(), &graph_frame_), "buffer_alloc");
+    check(c, usx_buffer_upload(c, graph_rf_, data.rf.data(), data.rf.size() * sizeof(float)),
+          "buffer_upload");
+    capture(reinterpret_cast<std::uintptr_t>(graph_rf_), reinterpret_cast<std::uintptr_t>(graph_frame_));
+  }
+  py::array_t<std::complex<float>> graph_frame() {
+    if (!graph_frame_) throw std::logic_error("graph_frame: no frame buffer (call capture_host first)");
+    py::array_t<std::complex<float>> a({bplan_.desc.n_axis0, bplan_.desc.n_axis1});
+    usx_ctx* c = ctx_.get();
+    check(c, usx_buffer_download(c, a.mutable_data(), graph_frame_, sizeof(usx_c32) * frame_pixels()),
+          "buffer_download");
+    return a;
+  }
   void drop_graph() {
-    if (graph_) usx_plan_destroy(ctx_.get(), graph_);
+    usx_ctx* c = ctx_.get();
+    if (graph_) usx_plan_destroy(c, graph_);
+    if (graph_rf_) usx_buffer_free(c, graph_rf_);
+    if (graph_frame_) usx_buffer_free(c, graph_frame_);
     graph_ = nullptr;
+    graph_rf_ = graph_frame_ = nullptr;
   }
   void synchronize() {
     usx_ctx* c = ctx_.get();
@@ -371,6 +399,8 @@ class DeviceRunner {
   BeamformPlan bplan_;
   GpuContext ctx_;
   usx_plan* graph_ = nullptr;
+  void* graph_rf_ = nullptr;     // device copies owned for capture_host
+  void* graph_frame_ = nullptr;
 };
 
 // A fixed sequence of RF frames, each with its own probe, grid, transmits and
@@ -1092,6 +1122,8 @@ PYBIND11_MODULE(_usx, m) {
       .def("capture", &DeviceRunner::capture, py::arg("input_ptr"), py::arg("frame_ptr"),
            py::call_guard<py::gil_scoped_release>())
       .def("run_graph", &DeviceRunner::run_graph, py::call_guard<py::gil_scoped_release>())
+      .def("capture_host", &DeviceRunner::capture_host, py::arg("channel_data"))
+      .def("graph_frame", &DeviceRunner::graph_frame)
       .def_property_readonly("stream", &DeviceRunner::stream)
       .def("record", &DeviceRunner::record, py::arg("slot"))
       .def("elapsed", &DeviceRunner::elapsed, py::arg("a"), py::arg("b"))
