@@ -379,6 +379,8 @@ def main():
     ap.add_argument("--pmc-workload", type=int, default=0, metavar="N",
                     help="run only N MV frames and a 256 MiB calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--skip-stream", action="store_true", help="omit the uneven-stream leg")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run only the uneven-stream leg and print its result (for profiling)")
     ap.add_argument("--stream-frames", type=int, default=64)
     ap.add_argument("--tile-cache", default=None,
                     help="autotuner cache file (default: the package's usx/data/tile_cache.json)")
@@ -391,6 +393,9 @@ def main():
                                    "data-parallel over frames and is measured per GPU",
                           "n_gpus": a.gpus}))
         return 2
+    if a.stream_only:
+        print(json.dumps(run_stream(a.stream_frames, a.tile_cache)))
+        return 0
 
     import torch
     import usx
