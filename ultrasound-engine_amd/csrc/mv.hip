@@ -949,6 +949,7 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
       f4 acc;
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = tri[2 * at(16 * mt + 4 * lq + i, j0 + c) + part];
+#pragma unroll 4  // the steps' LDS reads in flight together (j0 / 2 is a multiple of 4)
       for (int st = 0; st < (j0 >> 1); ++st) {
         const int k = 2 * st + (lq >> 1);
         const float a = tri[2 * at(16 * mt + lr, k) + kp];
@@ -1263,22 +1264,26 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         const float d = group_bcast<G>(sr[c], j, lane);
         ok = ok && (!col || d > 0.0f);
         // 1 / L_jj in one instruction (v_rsq_f32, ~1 ulp); L_jj itself is
-        // only stored, never read back.
-        const float rj = __builtin_amdgcn_rsqf(d);
+        // never needed (the pivot row stores 0 in its place). A group whose
+        // pixel has no column j takes 0, so its y stay finite and its b as they are.
+        const float rj = col ? __builtin_amdgcn_rsqf(d) : 0.0f;
         const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
         const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
         const bool piv = col && sub == j, below = col && sub > j;
-        const float ar = sr[c] * rj, ai = si[c] * rj;
-        lr[c] = piv ? d * rj : ar;
-        li[c] = piv ? 0.0f : ai;
+        // rows not below the pivot take l = 0: their entries of this column are
+        // never read (rows above j are not stored, and the factor's diagonal is
+        // not read back), so b -= l y leaves them as they are without a select
+        const float ar = below ? sr[c] * rj : 0.0f, ai = below ? si[c] * rj : 0.0f;
+        lr[c] = ar;
+        li[c] = ai;
         const float n1r = fmaf(-ar, y1r, fmaf(ai, y1i, b1r));  // b -= l y
         const float n1i = fmaf(-ar, y1i, fmaf(-ai, y1r, b1i));
         const float n2r = fmaf(-ar, y2r, fmaf(ai, y2i, b2r));
         const float n2i = fmaf(-ar, y2i, fmaf(-ai, y2r, b2i));
-        b1r = piv ? y1r : (below ? n1r : b1r);
-        b1i = piv ? y1i : (below ? n1i : b1i);
-        b2r = piv ? y2r : (below ? n2r : b2r);
-        b2i = piv ? y2i : (below ? n2i : b2i);
+        b1r = piv ? y1r : n1r;
+        b1i = piv ? y1i : n1i;
+        b2r = piv ? y2r : n2r;
+        b2i = piv ? y2i : n2i;
       }
       if (sub >= j0 && sub < L) {
 #pragma unroll
