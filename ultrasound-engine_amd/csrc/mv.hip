@@ -1157,10 +1157,9 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int t = t0 + u;
-        if (sub + t < L) {
-          const float vr = cr * inv;
-          A[pu[u]] = make_float2(vr, sub == 0 ? 0.0f : ci * inv);
-        }
+        // (the main diagonal's imaginary part, rounding noise, is never read:
+        // the Cholesky takes the pivot's real part only)
+        if (sub + t < L) A[pu[u]] = make_float2(cr * inv, ci * inv);
         cr += d[u].x;
         ci += d[u].y;
       }
