@@ -403,9 +403,12 @@ __device__ __forceinline__ void block_update_full(float* A, bool packed, int lan
   const int lr = lane & 15, lq = lane >> 4;
   const int c = lr >> 1, part = lr & 1;
   const int kp = lq & 1;
+  // -B[2k + kp][2c + part]: one component of b times a lane constant sign
+  const int comp = part ^ kp;
+  const float sgn = (part == 1 && kp == 0) ? 1.0f : -1.0f;
   const int col = j0 + c;  // < P: j0 < L <= P, both multiples of 8
   const int t1 = (L + 15) >> 4;
-  const float* bq = A + base(col);
+  const float* bq = A + base(col) + comp;
   for (int mt = j0 >> 4; mt < t1; ++mt) {
     f4 acc;
 #pragma unroll
@@ -414,11 +417,7 @@ __device__ __forceinline__ void block_update_full(float* A, bool packed, int lan
 #pragma unroll 2
     for (int st = 0; st < (j0 >> 1); ++st) {
       const int k = 2 * st + (lq >> 1);
-      const float a = arow[2 * k];
-      const float br = bq[2 * k], bi = bq[2 * k + 1];
-      // -B[2k + kp][2c + part]
-      const float nb = part == 0 ? -(kp ? bi : br) : (kp ? -br : bi);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, nb, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[2 * k], sgn * bq[2 * k], acc, 0, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -938,7 +937,11 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
   constexpr int LP = 64 / G;
   const int lr = lane & 15, lq = lane >> 4;
   const int c = lr >> 1, part = lr & 1;
-  const int kp = lq & 1;
+  const int kp = lq & 1, kh = lq >> 1;  // this lane's complex column k = 2 st + kh
+  // -B[2k + kp][2c + part] = -b_r, -b_i, +b_i, -b_r for (part, kp) = (0,0), (0,1),
+  // (1,0), (1,1): one component of b times a lane constant sign
+  const int comp = part ^ kp;
+  const float sgn = (part == 1 && kp == 0) ? 1.0f : -1.0f;
   const int t0 = j0 >> 4, t1 = (Lw + 15) >> 4;
   auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
 #pragma unroll
@@ -949,14 +952,19 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
       f4 acc;
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = tri[2 * at(16 * mt + 4 * lq + i, j0 + c) + part];
+      // float index of L(row, k) part kp, advanced incrementally: at(i, k + 2) -
+      // at(i, k) = 2P - 2k - 3, which falls by 4 per step; L(j0 + c, k) part comp
+      // sits a fixed distance db away (at(i, k) - at(i', k) = i - i')
+      const int row = 16 * mt + lr;
+      int ia = 2 * at(row, kh) + kp;
+      int step = 2 * (2 * P - 2 * kh - 3);
+      const int db = 2 * (j0 + c - row) + comp - kp;
 #pragma unroll 4  // the steps' LDS reads in flight together (j0 / 2 is a multiple of 4)
       for (int st = 0; st < (j0 >> 1); ++st) {
-        const int k = 2 * st + (lq >> 1);
-        const float a = tri[2 * at(16 * mt + lr, k) + kp];
-        const float br = tri[2 * at(j0 + c, k)], bi = tri[2 * at(j0 + c, k) + 1];
-        // -B[2k + kp][2c + part]
-        const float nb = part == 0 ? -(kp ? bi : br) : (kp ? -br : bi);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, nb, acc, 0, 0, 0);
+        const float a = tri[ia], b = tri[ia + db];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sgn * b, acc, 0, 0, 0);
+        ia += step;
+        step -= 8;
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
