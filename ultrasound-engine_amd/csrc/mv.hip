@@ -995,10 +995,14 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
   const int lr = lane & 15, lq = lane >> 4;
   const int c = lr >> 1, part = lr & 1;
   const int hi = lq >> 1, comp = lq & 1;  // this lane's k-part: window and component
+  // B[k-part][2c + part] = Re: [w_r w_i], Im: [-w_i w_r], the low window's
+  // negated: component part ^ comp of w times a lane constant sign
+  const int bcomp = part ^ comp;
+  const float bsgn = ((part == 1 && comp == 0) ? -1.0f : 1.0f) * (hi ? 1.0f : -1.0f);
   auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
-    const float2* rows = reinterpret_cast<const float2*>(smem + gg * group_bytes + rows_off);
+    const float* rows = reinterpret_cast<const float*>(smem + gg * group_bytes + rows_off);
     float* tri = reinterpret_cast<float*>(smem + gg * group_bytes + tri_off);
     const int Lg = G == 1 ? L : __shfl(L, gg * LP, 64);
     const int nsg = G == 1 ? ns : __shfl(ns, gg * LP, 64);  // every lane takes part
@@ -1008,21 +1012,15 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
       const int i = min(16 * mt + lr, Lg - 2) + off;
       float a[NR];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const float2 v = rows[r * RS + i];
-        a[r] = comp ? v.y : v.x;
-      }
+      for (int r = 0; r < NR; ++r) a[r] = rows[2 * (r * RS + i) + comp];
       const int cbn = min(2 * mt + 2, c1);  // column blocks reaching the tile's rows
       for (int cb = 0; cb < cbn; ++cb) {
         const int j = min(8 * cb + c, Lg - 2) + off;
         f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          const float2 w = rows[r * RS + j];
-          // B[k-part][2c + part]: Re: [w_r w_i], Im: [-w_i w_r]; low window negated
-          const float b = part == 0 ? (comp ? w.y : w.x) : (comp ? w.x : -w.y);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], hi ? b : -b, acc, 0, 0, 0);
-        }
+        for (int r = 0; r < NR; ++r)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bsgn * rows[2 * (r * RS + j) + bcomp],
+                                                     acc, 0, 0, 0);
         const int col = 8 * cb + c;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
