@@ -427,6 +427,41 @@ __device__ __forceinline__ void block_update_full(float* A, bool packed, int lan
   }
 }
 
+// Walk increments of the general path on MFMA (as walk_increments_mfma of the
+// fast path, for one pixel per wavefront): D(i, k) = sum_r x_r[i + n_sub]
+// conj(x_r[k + n_sub]) - x_r[i] conj(x_r[k]) for k <= i < L into the lower
+// triangle (packed by rows, or rows of stride P + 1), from the nrow staged
+// rows of stride na; indices clamped to L - 2 (reads stay in the staged
+// entries 0 .. na - 1; row L - 1 gets values the walk overwrites unread).
+template <int P>
+__device__ __forceinline__ void walk_increments_full(float* A, bool packed, const float* rows,
+                                                     int na, int nrow, int ns, int L, int lane) {
+  auto base = [packed](int r) { return 2 * (packed ? (r * (r + 1)) >> 1 : r * (P + 1)); };
+  const int lr = lane & 15, lq = lane >> 4;
+  const int c = lr >> 1, part = lr & 1;
+  const int hi = lq >> 1, comp = lq & 1;
+  const int bcomp = part ^ comp;
+  const float bsgn = ((part == 1 && comp == 0) ? -1.0f : 1.0f) * (hi ? 1.0f : -1.0f);
+  const int off = hi ? ns : 0;
+  const int t1 = (L + 15) >> 4, c1 = (L + 7) >> 3;
+  for (int mt = 0; mt < t1; ++mt) {
+    const float* ra = rows + 2 * (min(16 * mt + lr, L - 2) + off) + comp;
+    const int cbn = min(2 * mt + 2, c1);
+    for (int cb = 0; cb < cbn; ++cb) {
+      const float* rb = rows + 2 * (min(8 * cb + c, L - 2) + off) + bcomp;
+      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int r = 0; r < nrow; ++r)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[2 * r * na], bsgn * rb[2 * r * na], acc, 0, 0, 0);
+      const int col = 8 * cb + c;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * mt + 4 * lq + e;
+        if (row >= col && row < L) A[base(row) + 2 * col + part] = acc[e];
+      }
+    }
+  }
+}
+
 template <int P, int PREC>
 __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restrict__ Y,
                                                const uint32_t* __restrict__ mask,
@@ -493,7 +528,8 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
       // FP32 inputs: the subarray shift recursion along the sub-diagonals,
       // lane owns m = lane + 64 q,
       //   R(m,0) = sum_r sum_{a<ns} x_r[a+m] conj(x_r[a]),
-      //   R(m+t+1,t+1) = R(m+t,t) + sum_r x_r[m+t+ns] conj(x_r[t+ns]) - x_r[m+t] conj(x_r[t]),
+      //   R(m+t+1,t+1) = R(m+t,t) + D(m+t,t),
+      //   D(i,k) = sum_r x_r[i+ns] conj(x_r[k+ns]) - x_r[i] conj(x_r[k]),
       // O(L n_snap + L^2 nrow) products instead of the rank-k update's
       // O(L^2 n_snap). Lanes past their diagonal's end read clamped indices
       // and store nothing.
@@ -516,32 +552,39 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
           }
         }
       }
-      for (int t = 0;; ++t) {
+      // Increments D on MFMA into the lower triangle, then lane m walks its
+      // sub-diagonals in place: at (m+t, t) it reads D, stores R and adds;
+      // the reads of 4 steps go out before their stores (a lane touches only
+      // its own diagonals).
+      walk_increments_full<P>(reinterpret_cast<float*>(A), packed,
+                              reinterpret_cast<const float*>(rows), na, nrow, ns, L, lane);
+      __syncthreads();
+      for (int t0 = 0; t0 < L; t0 += 4) {
+        float2 d[NS][4];
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-          const int m = lane + 64 * q, i = m + t;
-          if (i < L) {
-            const float2 v = make_float2(cr[q] * inv, ci[q] * inv);
-            if (m == 0) {
-              dg[t] = make_float2(v.x, 0.0f);
-            } else {
-              A[ix(i, t)] = v;                                   // factor input, lower
-              if (!packed) A[t * S + i] = make_float2(v.x, -v.y);  // R, upper
-            }
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = min(lane + 64 * q + t0 + u, L - 1);
+            d[q][u] = A[ix(i, min(t0 + u, i))];  // past the diagonal's end: read, never stored
           }
-        }
-        if (t + 1 >= L) break;
-        const float2* x = rows;
-        for (int r = 0; r < nrow; ++r, x += na) {
-          const float2 hk = x[t + ns], lk = x[t];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = t0 + u;
 #pragma unroll
           for (int q = 0; q < NS; ++q) {
-            const int il = min(lane + 64 * q + t, L - 2);
-            const float2 hi = x[il + ns], lo = x[il];
-            cr[q] = fmaf(hi.x, hk.x, fmaf(hi.y, hk.y, cr[q]));
-            ci[q] = fmaf(hi.y, hk.x, fmaf(-hi.x, hk.y, ci[q]));
-            cr[q] = fmaf(-lo.x, lk.x, fmaf(-lo.y, lk.y, cr[q]));
-            ci[q] = fmaf(-lo.y, lk.x, fmaf(lo.x, lk.y, ci[q]));
+            const int m = lane + 64 * q, i = m + t;
+            if (i < L) {
+              const float2 v = make_float2(cr[q] * inv, ci[q] * inv);
+              if (m == 0) {
+                dg[t] = make_float2(v.x, 0.0f);
+              } else {
+                A[ix(i, t)] = v;                                   // factor input, lower
+                if (!packed) A[t * S + i] = make_float2(v.x, -v.y);  // R, upper
+              }
+            }
+            cr[q] += d[q][u].x;
+            ci[q] += d[q][u].y;
           }
         }
       }
