@@ -1115,7 +1115,9 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     const int L = min(max(M.subarray, 2), max(2, na / 2));
     const int ns = na - L + 1;
     const float inv = 1.0f / static_cast<float>(nrow * ns);
-    const int Lw = wave_max(L), nsw = wave_max(ns);  // uniform loop bounds
+    // uniform loop bounds, kept scalar
+    const int Lw = __builtin_amdgcn_readfirstlane(wave_max(L));
+    const int nsw = __builtin_amdgcn_readfirstlane(wave_max(ns));
 
     // ---- covariance: first column, then the sub-diagonal walks ---------------
     // Loops have uniform bounds and fixed row counts (NR rows, the missing
@@ -1270,15 +1272,19 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         sr[c] = 0.0f;
         si[c] = 0.0f;
       }
-      if (sub >= j0 && sub < L) {
+      {
+        // branch-free: every lane reads (its own entry, or entry 0) and keeps
+        // the value where its row reaches the column
+        const bool rowon = sub >= j0 && sub < L;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-          if (j0 + c <= sub) {
-            const float2 v = A[at(sub, j0 + c)];
-            sr[c] = v.x;
-            si[c] = v.y;
-          }
+          const bool on = rowon && j0 + c <= sub;
+          const float2 v = A[on ? at(sub, j0 + c) : 0];
+          sr[c] = on ? v.x : 0.0f;
+          si[c] = on ? v.y : 0.0f;
         }
+      }
+      if (sub >= j0 && sub < L) {
         if constexpr (!kMfmaUpdate) {
 #pragma unroll 2
           for (int k = 0; k < j0; ++k) {
