@@ -229,10 +229,10 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
   const float trx = rx_time(A.ex[ch], p, A.c);
   const float nm1 = static_cast<float>(A.n_s - 1);
   float2 acc = make_float2(0.0f, 0.0f);
-  // slots in groups of four: one uniform branch per group, and inside it all
-  // four loads issued together (a slot past nsl reads slot 0's trace and is
-  // not added)
-  constexpr int G = NE == 12 ? 6 : (NE < 4 ? NE : 4);
+  // slots in groups of G: one uniform branch per group, and inside it all
+  // the group's loads issued together (a slot past nsl reads slot 0's trace
+  // and is not added; slots past NE are not formed at all)
+  constexpr int G = (NE == 11 || NE == 12) ? 6 : (NE < 4 ? NE : 4);
 #pragma unroll
   for (int s0 = 0; s0 < NE; s0 += G) {
     if (s0 < nsl) {
@@ -240,6 +240,7 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
 #pragma unroll
       for (int q = 0; q < G; ++q) {
         const int s = s0 + q;
+        if (s >= NE) break;
         const bool on = s < nsl;
         const float idx = (sl[on ? s : 0].tau + trx - A.t0) * A.fs;
         const int ev = SEL ? sl[on ? s : 0].ev : ev_first + (on ? s : 0);
@@ -253,7 +254,7 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
 #pragma unroll
       for (int q = 0; q < G; ++q) {
         const int s = s0 + q;
-        if (s < nsl) {
+        if (s < NE && s < nsl) {
           acc.x = fmaf(v[q].x, sl[s].tr, fmaf(-v[q].y, sl[s].ti, acc.x));
           acc.y = fmaf(v[q].x, sl[s].ti, fmaf(v[q].y, sl[s].tr, acc.y));
         }
@@ -562,6 +563,8 @@ static void launch_fused_iq(const BfArgs& A, const void* data, float2* frame, in
     launch_fused_ne<4, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
   else if (n_slots <= 8)
     launch_fused_ne<8, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
+  else if (n_slots == 11)  // 11 transmits: no idle 12th slot in the second load group
+    launch_fused_ne<11, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
   else if (n_slots <= 12)
     launch_fused_ne<12, IQ>(A, data, frame, ev_first, n_slots, select, accumulate, comb, cf, st);
   else
@@ -594,6 +597,8 @@ static void launch_channels_iq(const BfArgs& A, const void* data, float2* Y, uin
     launch_channels_ne<4, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
   else if (n_slots <= 8)
     launch_channels_ne<8, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
+  else if (n_slots == 11)
+    launch_channels_ne<11, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
   else if (n_slots <= 12)
     launch_channels_ne<12, IQ>(A, data, Y, mask, ev_first, n_slots, select, first, rect, st);
   else
