@@ -34,8 +34,9 @@
 //   1. active list; centre and +-range_avg neighbour vectors staged in LDS
 //      (neighbours clipped at the grid edge, beamformer.cpp:159-161);
 //   2. covariance R = sum over (depth, subarray start) of x x^H: with FP32
-//      inputs by the shift recursion above, lane m walking sub-diagonals
-//      m + 64 q; with BF16/FP16 inputs as a dense rank-k update on MFMA,
+//      inputs by the shift recursion above (increments on MFMA), lane m
+//      walking sub-diagonals m + 64 q; with BF16/FP16 inputs as a dense
+//      rank-k update on MFMA,
 //      complex expanded to real:
 //        Re R = [xr xi][xr xi]^T,  Im R = [xi -xr][xr xi]^T,
 //      v_mfma_f32_16x16x4_f32, two snapshots per instruction (k = re/im of
@@ -952,7 +953,8 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
 //     R(i,k) = sum_r sum_{a<n_sub} x_r[a+i] conj(x_r[a+k]),
 //   R(i+1,k+1) = R(i,k) + sum_r (x_r[i+n_sub] conj(x_r[k+n_sub]) - x_r[i] conj(x_r[k])),
 //   so lane m forms the first-column entry R(m,0) directly (n_snap products)
-//   and walks the m-th sub-diagonal adding 2 nrow products per step:
+//   and walks the m-th sub-diagonal, the increments of every step formed at
+//   once on MFMA (walk_increments_mfma), one add per step:
 //   O(L n_snap + L^2 nrow) per pixel instead of O(L^2 n_snap).
 // * The forward substitutions run inside the Cholesky column loop
 //   (column-oriented: once column j of the factor is known, y_j = b_j / L_jj
