@@ -317,21 +317,20 @@ __global__ void mv_zero_counts(int* counts) {
   if (threadIdx.x < kMaxBuckets) counts[threadIdx.x] = 0;
 }
 
-// Slot in bucket b's list, one atomic per (wavefront, bucket): lanes of a
-// wavefront that join the same bucket take consecutive slots.
+// Slot in bucket b's list: the workgroup's lanes count per bucket in LDS
+// (their local slots), then one global atomic per (workgroup, bucket) claims
+// a base; a workgroup's pixels of one bucket take consecutive slots. Every
+// thread of the workgroup calls this (barriers).
 __device__ __forceinline__ int bucket_slot(int* counts, int b, bool take) {
-  int slot = 0;
-  for (int q = 0; q < kMaxBuckets; ++q) {
-    const unsigned long long m = __ballot(take && b == q);
-    if (m == 0) continue;
-    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-    int base = 0;
-    if (static_cast<int>(threadIdx.x & 63) == leader) base = atomicAdd(&counts[q], __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (take && b == q)
-      slot = base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
-  }
-  return slot;
+  __shared__ int hist[kMaxBuckets], base[kMaxBuckets];
+  const int t = threadIdx.x;
+  if (t < kMaxBuckets) hist[t] = 0;
+  __syncthreads();
+  const int local = take ? atomicAdd(&hist[b], 1) : 0;
+  __syncthreads();
+  if (t < kMaxBuckets) base[t] = hist[t] ? atomicAdd(&counts[t], hist[t]) : 0;
+  __syncthreads();
+  return take ? base[b] + local : 0;
 }
 
 __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __restrict__ Y,
