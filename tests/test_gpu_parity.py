@@ -338,6 +338,11 @@ def test_mv_192_channels_subarray_96(usx_mod, port, O):
     _, exw = port.beamform(s2, txs, np.asarray(iq.iq), case.gspec, O.Bf.of(bfc),
                            fdem=iq.demodulation_frequency, weights=True)
     assert rel_l2(w.reshape(-1, 192), exw["weights"]) < TOL_MV
+    # and with the refinement step: the full matrix in LDS (its upper triangle
+    # keeps R for the residual) instead of the packed lower triangle
+    bfr = bf_config(usx_mod, "MV", subarray=96, f_number=1.0, refine=True)
+    got_r = np.asarray(usx_mod.Beamformer(bfr).beamform(iq, case.grid).data)
+    assert rel_l2(got_r, ref) < TOL_MV, rel_l2(got_r, ref)
 
 
 def _pc(usx, bfc):
