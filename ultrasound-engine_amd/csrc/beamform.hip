@@ -407,7 +407,9 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   const size_t pix0 = static_cast<size_t>(i0) * A.g.n1 + (j - lane);
   if (lo <= hi) {
     for (int c0 = lo & ~31; c0 <= hi; c0 += 32) {
-      // each wave: channels c0 + w + 4k of this lane's pixel
+      // each wave: channels c0 + w + 4k of this lane's pixel; their mask bits
+      // (bit cc of word c0 / 32) gathered in a register, one LDS OR per block
+      uint32_t mybits = 0u;
       for (int cc = w; cc < 32; cc += 4) {
         const int ch = c0 + cc;
         float2 y = make_float2(0.0f, 0.0f);
@@ -415,13 +417,14 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
           float a = aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha);
           if (rect) a = a > 0.0f ? 1.0f : 0.0f;
           if (a > 0.0f) {
-            atomicOr(&bits[lane][ch >> 5], 1u << (ch & 31));
+            mybits |= 1u << cc;
             y = select ? channel_value<NE, IQ, true>(A, data, p, sl, nsl, ch, a, ev_first)
                        : channel_value<NE, IQ, false>(A, data, p, sl, nsl, ch, a, ev_first);
           }
         }
         tile[lane][cc] = y;
       }
+      if (mybits) atomicOr(&bits[lane][c0 >> 5], mybits);
       __syncthreads();
       // write the block: 64 pixel rows x 32 channels, 4 lanes per 32-byte piece
       const int ncols = min(32, A.n_ch - c0);
