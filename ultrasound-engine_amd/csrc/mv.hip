@@ -221,6 +221,11 @@ __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
 
 // stage_pixel for one pixel per lane group, with the depth count fixed at
 // NR = 2 range_avg + 1 and the row stride at RS >= n_ch (compile-time, so
@@ -1162,8 +1167,31 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
             ai[0] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[0]));
           }
         }
-      } else {  // groups: clamp to each pixel's n_sub
-        for (int a = 0; a < nsw; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
+      } else {  // groups: pairs below every group's n_sub unclamped, then clamped
+        const int nsmin = __builtin_amdgcn_readfirstlane(wave_min(ns));
+        int a = 0;
+        for (; a + 1 < nsmin; a += 2) {
+          float2 p[NR][2], q[NR][2];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const float2* x = rows + r * RS;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              p[r][u] = x[a + u + m];
+              q[r][u] = x[a + u];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const float2 pp = p[r][u], qq = q[r][u];
+              ar[u] = fmaf(pp.x, qq.x, fmaf(pp.y, qq.y, ar[u]));
+              ai[u] = fmaf(pp.y, qq.x, fmaf(-pp.x, qq.y, ai[u]));
+            }
+          }
+        }
+        for (; a < nsw; a += 2) {  // R(m,0) += x[a+m] conj(x[a])
           float2 p[NR][2], q[NR][2];
 #pragma unroll
           for (int r = 0; r < NR; ++r) {
