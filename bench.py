@@ -31,8 +31,14 @@ over the profiled timed region.
 
 cpu_baseline: the reference's own core (oracle/_ref, built from its sources)
 or else this repository's restatement (oracle/), timed on this host's cores
-on a bounded sample (64 lateral lines of the same depth range), scaled to a
-full frame.
+(OMP_NUM_THREADS, else the CPUs this process may use) on a bounded sample:
+MV on 64 of the 256 lateral lines at full depth, one warm-up then the median
+of 3, scaled x4; beside it a 1-thread MV sample, DAS / DAS+CF / DMAS full
+frames (median of 5) and the CPU model and SMT state.
+
+parity: the relative L2 of the timed GPU frames (DAS, DAS + CF, DMAS on every
+pixel; MV on the 64 lines the CPU computed) against the reference's frames of
+the same RF. Above 1e-5 (DAS family) or 1e-4 (MV) the bench exits with 3.
 """
 
 from __future__ import annotations
@@ -244,11 +250,62 @@ def doppler_leg(usx, torch, frame, grid, steps, warmup, n_frames=16, prf=5000.0)
     return out, ens, dc
 
 
-def cpu_das_and_doppler(usx, probe, med, acq, txs, raw, grid, ens, dc, prf=5000.0):
-    """The reference's CPU DAS process_frame on the full frame (its OpenMP
-    threads) and its colour flow on the same 16-frame ensemble as the GPU leg
-    (single-threaded in the reference), each timed once after a warm-up of
-    the smaller one."""
+def cpu_info():
+    """Host CPU facts for the baseline: model, logical CPUs usable by this
+    process, and whether SMT is on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable, "smt": smt}
+
+
+def _omp_threads(n):
+    """Set the OpenMP thread count of the CPU checkers' parallel regions
+    (they share the process's libgomp)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def _median_s(fn, reps):
+    """One untimed warm-up call, then the median wall time of `reps` calls;
+    returns (median seconds, every time, the last result)."""
+    fn()
+    times, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times, out
+
+
+def cpu_legs(usx, probe, med, acq, txs, raw, ens, dc, threads, prf=5000.0):
+    """The reference's CPU pipeline (its own core built from its sources,
+    oracle/_ref, else this repository's restatement) on the same RF:
+
+    * MV process_frame on 64 of the 256 lateral lines (lines 2, 6, ..., 254 of
+      the benchmark grid, at its axis values), warm-up + median of 3, at
+      `threads` OpenMP threads, scaled x4 to the full frame; and on 4 lines
+      (34, 98, 162, 226) with 1 thread, median of 3, scaled x64;
+    * DAS, DAS + coherence factor and DMAS process_frame on the full frame,
+      warm-up + median of 5 at `threads`, and DAS with 1 thread, median of 3;
+    * colour flow on the bench's 16-frame ensemble (single-threaded, as the
+      reference's colour flow has no OpenMP loop), warm-up + median of 3.
+
+    Returns the MV baseline, the DAS baseline, the colour-flow time and the
+    reference frames (DAS, DAS + CF, DMAS full; MV on its 64 lines)."""
     from oracle import oracle as O
 
     flavour = "reference" if O.available("reference") else "port"
@@ -256,21 +313,62 @@ def cpu_das_and_doppler(usx, probe, med, acq, txs, raw, grid, ens, dc, prf=5000.
     s = O.Setup.of(probe, med, acq)
     otx = [O.Tx.of(t) for t in txs]
     half = 0.5 * probe.aperture_width()
-    gs = O.GridSpec(0, -half, half, 256, 0.005, 0.050, 640)
-    bf = O.Bf(f_number=1.5, combiner=0)
     rf = np.asarray(raw.rf)
-    t0 = time.perf_counter()
-    orc.process_frame(s, otx, rf, gs, O.Demod(decimation=4), bf)
-    das_s = time.perf_counter() - t0
+    dm = O.Demod(decimation=4)
+
+    def frame(comb, cf=0, first=0, step=0):
+        gs = O.GridSpec(0, -half, half, 256, 0.005, 0.050, 640, line_first=first, line_step=step)
+        bf = O.Bf(f_number=1.5, combiner=comb, coherence_factor=cf, mv_subarray=64,
+                  mv_diagonal_loading=1e-2, mv_range_average=2)
+        return lambda: orc.process_frame(s, otx, rf, gs, dm, bf)[0]
+
+    refs = {}
+    _omp_threads(threads)
+    mv_s, mv_all, refs["mv"] = _median_s(frame(2, 0, 2, 4), 3)
+    das_s, das_all, refs["das"] = _median_s(frame(0), 5)
+    refs["das_cf"] = frame(0, 1)()
+    refs["dmas"] = frame(1)()
+    _omp_threads(1)
+    mv1_s, mv1_all, _ = _median_s(frame(2, 0, 34, 64), 3)
+    das1_s, das1_all, _ = _median_s(frame(0), 3)
     d = O.Doppler.of(dc)
-    t0 = time.perf_counter()
-    orc.color_flow(gs, ens, 5.0e6, prf, 1540.0, d)
-    cf_s = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return ({"value": 1.0 / das_s, "unit": "frames/s", "cores": threads, "kind": flavour,
-             "sample": "DAS process_frame, full 256 x 640 frame"},
-            {"ms": cf_s * 1e3, "cores": 1, "kind": flavour,
-             "sample": f"color_flow on the {ens.shape[0]}-frame 256 x 640 ensemble"})
+    cf_s, _, _ = _median_s(lambda: orc.color_flow(O.GridSpec(0, -half, half, 256, 0.005, 0.050, 640),
+                                                  ens, 5.0e6, prf, 1540.0, d), 3)
+    _omp_threads(threads)
+    host = cpu_info()
+    mv = {"value": 1.0 / (mv_s * 4.0), "unit": "frames/s", "cores": threads, "kind": flavour,
+          "sample": "MV process_frame on 64 of the 256 lateral lines (every 4th) x 640 depths, "
+                    "one warm-up then the median of 3 runs, scaled x4 to the full frame",
+          "sample_seconds": [round(t, 3) for t in mv_all],
+          "one_thread": {"value": 1.0 / (mv1_s * 64.0), "unit": "frames/s", "cores": 1,
+                         "sample": "4 lateral lines x 640 depths, median of 3, scaled x64",
+                         "sample_seconds": [round(t, 3) for t in mv1_all]},
+          "host": host}
+    das = {"value": 1.0 / das_s, "unit": "frames/s", "cores": threads, "kind": flavour,
+           "sample": "DAS process_frame, full 256 x 640 frame, one warm-up then the median of 5",
+           "sample_seconds": [round(t, 4) for t in das_all],
+           "one_thread": {"value": 1.0 / das1_s, "unit": "frames/s", "cores": 1,
+                          "sample_seconds": [round(t, 4) for t in das1_all]},
+           "host": host}
+    cf = {"ms": cf_s * 1e3, "cores": 1, "kind": flavour,
+          "sample": f"color_flow on the {ens.shape[0]}-frame 256 x 640 ensemble, median of 3"}
+    return mv, das, cf, refs
+
+
+def parity(frames, refs):
+    """Relative L2 of the timed GPU frames against the reference's CPU frames
+    of the same RF: DAS, DAS + CF and DMAS on every pixel (bar 1e-5), MV on
+    the reference's 64 lines (bar 1e-4)."""
+    def rel(a, b):
+        a = np.asarray(a, np.complex128)
+        b = np.asarray(b, np.complex128)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {"das": rel(frames["das"], refs["das"]), "das_cf": rel(frames["das_cf"], refs["das_cf"]),
+           "dmas": rel(frames["dmas"], refs["dmas"]), "mv": rel(frames["mv"][2::4], refs["mv"]),
+           "tolerance": {"das": 1e-5, "das_cf": 1e-5, "dmas": 1e-5, "mv": 1e-4},
+           "mv_lines": "2, 6, ..., 254 of 256"}
+    out["pass"] = all(out[k] < out["tolerance"][k] for k in ("das", "das_cf", "dmas", "mv"))
+    return out
 
 
 def time_host(usx, raw, grid, pc, steps):
@@ -281,31 +379,6 @@ def time_host(usx, raw, grid, pc, steps):
     for _ in range(steps):
         pipe.process_frame(raw)
     return (time.perf_counter() - t0) * 1e3 / steps
-
-
-def cpu_baseline(usx, probe, med, acq, txs, raw, n_lines=128):
-    """Reference CPU MV on a bounded sample (n_lines lateral lines over the
-    full width and depth range), scaled to the 256-line frame."""
-    from oracle import oracle as O
-
-    flavour = "reference" if O.available("reference") else "port"
-    orc = O.load(flavour)
-    s = O.Setup.of(probe, med, acq)
-    otx = [O.Tx.of(t) for t in txs]
-    half = 0.5 * probe.aperture_width()
-    gs = O.GridSpec(0, -half, half, n_lines, 0.005, 0.050, 640)
-    bf = O.Bf(f_number=1.5, combiner=2, mv_subarray=64, mv_diagonal_loading=1e-2,
-              mv_range_average=2)
-    rf = np.asarray(raw.rf)
-    t0 = time.perf_counter()
-    _, st = orc.process_frame(s, otx, rf, gs, O.Demod(decimation=4), bf)
-    sample_s = time.perf_counter() - t0
-    frame_s = sample_s * (256.0 / n_lines)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": flavour,
-            "sample": f"MV frame on {n_lines} of 256 lateral lines x 640 depths "
-                      f"({sample_s:.1f} s), scaled x{256 // n_lines} to the full frame",
-            "sample_seconds": round(sample_s, 3)}
 
 
 def run_stream(n_frames, cache_path=None):
@@ -335,17 +408,21 @@ def run_stream(n_frames, cache_path=None):
     return res
 
 
-def pmc_workload(n_frames):
-    """The workload of the HBM-traffic counters: n MV frames of the benchmark
-    (device-resident RF) and, for calibration, one device copy of exactly
-    256 MiB. Under rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), traffic per
-    MV frame = counter sum over the MV-stage dispatches of a frame, times
-    (copy bytes / the copy's counter)."""
+def pmc_workload(n_frames, combiner="MV"):
+    """The workload of the counter passes: n frames of the benchmark through
+    `combiner` (MV, DAS, DMAS or Learned; device-resident RF) and, for
+    calibration, one device copy of exactly 256 MiB. Under rocprofv3 --pmc
+    FETCH_SIZE (then WRITE_SIZE), traffic per frame = counter sum over a
+    frame's dispatches, times (copy bytes / the copy's counter)."""
     import torch
     import usx
     probe, med, acq, txs, ph, raw, grid = make_workload(usx)
-    pc = pipeline_config(usx, "MV")
+    pc = pipeline_config(usx, combiner)
     runner = usx.DeviceRunner(grid, pc, raw)
+    if combiner == "Learned":
+        from usx import learned as L
+        runner.set_apodization(L.load_checkpoint(os.path.join(
+            ROOT, "ultrasound-engine_amd", "usx", "data", "apod_mlp.npz"))[0])
     rf = torch.from_numpy(np.ascontiguousarray(np.asarray(raw.rf))).to("cuda:0")
     frame = torch.empty((grid.n_axis0, grid.n_axis1), dtype=torch.complex64, device="cuda:0")
     for _ in range(n_frames):
@@ -356,7 +433,7 @@ def pmc_workload(n_frames):
     torch.cuda.synchronize()
     dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"pmc workload: {n_frames} MV frames, one {src.numel() * 4} B device copy")
+    log(f"pmc workload: {n_frames} {combiner} frames, one {src.numel() * 4} B device copy")
 
 
 def measured_traffic():
@@ -378,6 +455,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--pmc-workload", type=int, default=0, metavar="N",
                     help="run only N MV frames and a 256 MiB calibration copy (for rocprofv3 --pmc)")
+    ap.add_argument("--pmc-combiner", default="MV", choices=["MV", "DAS", "DMAS", "Learned"],
+                    help="combiner of the --pmc-workload frames")
     ap.add_argument("--skip-stream", action="store_true", help="omit the uneven-stream leg")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the uneven-stream leg and print its result (for profiling)")
@@ -386,7 +465,7 @@ def main():
                     help="autotuner cache file (default: the package's usx/data/tile_cache.json)")
     a = ap.parse_args()
     if a.pmc_workload:
-        pmc_workload(a.pmc_workload)
+        pmc_workload(a.pmc_workload, a.pmc_combiner)
         return 0
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the imaging path is "
@@ -413,10 +492,12 @@ def main():
     log(f"  {mv_ms:.3f} ms/frame (events {mv_ev_ms:.3f}, graph {mv_graph_ms:.3f}), kernels {kern}")
     extras = {"mv_fps": 1e3 / mv_ms, "mv_fps_graph": 1e3 / mv_graph_ms}
     das_frame = None
+    timed_frames = {"mv": mv_frame}
     for name, comb, cf in [("das", "DAS", False), ("das_cf", "DAS", True), ("dmas", "DMAS", False)]:
         ms, ev_ms, k, fr, g_ms = time_device(usx, torch, raw, grid, pipeline_config(usx, comb, cf),
                                              max(a.steps, 100), a.warmup, profile=True,
                                              graph=name == "das")
+        timed_frames[name] = fr
         if name == "das":
             das_frame = fr
             extras["das_fps_graph"] = 1e3 / g_ms
@@ -473,18 +554,21 @@ def main():
         extras["stream"] = run_stream(a.stream_frames, a.tile_cache)
 
     cpu = None
+    par = None
     if not a.skip_cpu:
-        log("CPU baseline (bounded sample) ...")
-        cpu = cpu_baseline(usx, probe, med, acq, txs, raw)
-        log(f"  {cpu}")
+        threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+        log(f"CPU baseline: the reference's pipeline on {threads} threads and on 1 ...")
+        cpu, cpu_das, cpu_cf, refs = cpu_legs(usx, probe, med, acq, txs, raw, d_ens, d_cfg, threads)
+        log(f"  MV {cpu}")
+        log(f"  DAS {cpu_das}, colour flow {cpu_cf}")
         extras["mv_speedup_vs_cpu"] = (1e3 / mv_ms) / cpu["value"]
-        log("CPU DAS frame and colour flow (reference) ...")
-        cpu_das, cpu_cf = cpu_das_and_doppler(usx, probe, med, acq, txs, raw, grid, d_ens, d_cfg)
+        extras["mv_speedup_vs_cpu_1thread"] = (1e3 / mv_ms) / cpu["one_thread"]["value"]
         extras["cpu_das"] = cpu_das
         extras["das_speedup_vs_cpu"] = extras["das_fps"] / cpu_das["value"]
         extras["doppler"]["cpu_color_flow"] = cpu_cf
         extras["doppler"]["color_flow_speedup_vs_cpu"] = cpu_cf["ms"] / extras["doppler"]["color_flow_ms"]
-        log(f"  DAS {cpu_das}, colour flow {cpu_cf}")
+        par = parity(timed_frames, refs)
+        log(f"  parity of the timed frames vs the reference: {par}")
 
     line = {
         "metric": METRIC, "value": 1e3 / mv_ms, "unit": "frames/s", "n_gpus": 1,
@@ -495,9 +579,12 @@ def main():
                                "MV L=64 loading 1e-2 range +-2, f/1.5, 256x640 grid",
                    "grid": [256, 640], "channels": 128, "transmits": 11,
                    "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu, "extras": extras,
+        "roofline": roof, "cpu_baseline": cpu, "parity": par, "extras": extras,
     }
     print(json.dumps(line))
+    if par is not None and not par["pass"]:
+        log(f"FAIL: timed frames differ from the reference beyond the bars: {par}")
+        return 3
     return 0
 
 

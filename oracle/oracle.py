@@ -54,7 +54,8 @@ class OcTransmit(C.Structure):
 
 class OcGrid(C.Structure):
     _fields_ = [("type", _i), ("a0_min", _f), ("a0_max", _f), ("n0", _i),
-                ("a1_min", _f), ("a1_max", _f), ("n1", _i), ("apex", _f * 3)]
+                ("a1_min", _f), ("a1_max", _f), ("n1", _i), ("apex", _f * 3),
+                ("line_first", _i), ("line_step", _i)]
 
 
 class OcDemod(C.Structure):
@@ -137,10 +138,19 @@ class GridSpec:
     a1_max: float = 0.03
     n1: int = 32
     apex: tuple = (0.0, 0.0, 0.0)
+    line_first: int = 0      # with line_step > 0: only lines line_first + k line_step,
+    line_step: int = 0       # at the full grid's axis values
+
+    @property
+    def lines(self) -> int:
+        """Lateral lines the frames of this spec hold."""
+        if self.line_step <= 0:
+            return self.n0
+        return len(range(self.line_first, self.n0, self.line_step))
 
     def c(self) -> OcGrid:
         return OcGrid(self.type, self.a0_min, self.a0_max, self.n0, self.a1_min, self.a1_max,
-                      self.n1, (_f * 3)(*self.apex))
+                      self.n1, (_f * 3)(*self.apex), self.line_first, self.line_step)
 
 
 @dataclass
@@ -379,12 +389,12 @@ class Oracle:
         is_iq = int(np.iscomplexobj(data))
         data = data.astype(np.complex64 if is_iq else np.float32, copy=False)
         s, g, b = setup.c(), grid.c(), bf.c()
-        frame = np.zeros((grid.n0, grid.n1), np.complex64)
+        frame = np.zeros((grid.lines, grid.n1), np.complex64)
         extra = {}
         if mv_fp64 or weights or ych or info:
             if not self.has_ex:
                 raise OracleError(f"{self.flavour} library has no oc_beamform_ex")
-            npix, nch = grid.n0 * grid.n1, setup.n_elements
+            npix, nch = grid.lines * grid.n1, setup.n_elements
             w = np.zeros((npix, nch), np.complex64) if weights else None
             y = np.zeros((npix, nch), np.complex64) if ych else None
             inf = np.zeros((npix, 3), np.int32) if info else None
@@ -415,11 +425,17 @@ class Oracle:
                                         fr.ctypes.data_as(_fp), _f(fdem), out.ctypes.data_as(_fp)))
         return out
 
+    def validate_bf(self, bf: Bf) -> None:
+        """BeamformerConfig::validate as the constructor runs it; raises
+        ValueError with the reference's message."""
+        b = bf.c()
+        self._check(self.lib.oc_validate_bf(C.byref(b)))
+
     def process_frame(self, setup: Setup, txs, rf, grid: GridSpec, demod: Demod, bf: Bf):
         arr, keep = self._txs(txs)
         rf = np.ascontiguousarray(rf, np.float32)
         s, g, d, b = setup.c(), grid.c(), demod.c(), bf.c()
-        frame = np.zeros((grid.n0, grid.n1), np.complex64)
+        frame = np.zeros((grid.lines, grid.n1), np.complex64)
         stats = (C.c_double * 3)()
         self._check(self.lib.oc_process_frame(C.byref(s), arr, len(txs), _ptr(rf), C.byref(g),
                                               C.byref(d), C.byref(b), frame.ctypes.data_as(_fp),

@@ -59,6 +59,10 @@ typedef struct {
   float a1_min, a1_max;
   int n1;
   float apex[3];
+  /* A subset of the lateral lines of that grid, with the full grid's axis
+   * values: lines line_first, line_first + line_step, ... below n0
+   * (line_step 0: every line). Frames then hold that many lines. */
+  int line_first, line_step;
 } oc_grid;
 
 typedef struct {
@@ -134,10 +138,17 @@ int oc_demodulate(const oc_setup* s, const oc_transmit* txs, int n_tx, const flo
 
 /* Beamformer::beamform (beamformer.cpp:297-549). `data` is RF (float) when
  * is_iq == 0, else interleaved IQ; s->n_samples / s->sampling_frequency
- * describe `data`. frame_out interleaved, n0*n1 complex. */
+ * describe `data`. frame_out interleaved, n0*n1 complex. The config is
+ * treated as assigned after construction: it is not validated, and
+ * out-of-range fields meet the reference's clamps; only minimum variance with
+ * a negative range average is refused (returns 1). */
 int oc_beamform(const oc_setup* s, const oc_transmit* txs, int n_tx, int is_iq,
                 const float* data, float fdem, const oc_grid* g, const oc_bf* bf,
                 int event_begin, int event_end, float* frame_out);
+
+/* BeamformerConfig::validate as the Beamformer constructor runs it
+ * (beamformer.hpp:110, beamformer.cpp:218-225): 0 or 1 with the message. */
+int oc_validate_bf(const oc_bf* bf);
 
 /* ImagingPipeline::process_frame (pipeline.cpp:25-46). stats_out receives
  * {demodulate_ms, beamform_ms, total_ms}; may be NULL. */

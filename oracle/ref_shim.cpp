@@ -66,11 +66,26 @@ std::vector<usx::Transmit> to_txs(const oc_transmit* txs, int n) {
   for (int i = 0; i < n; ++i) v.push_back(to_tx(txs[i]));
   return v;
 }
-usx::ScanGrid to_grid(const oc_grid* g) {
+usx::ScanGrid full_grid(const oc_grid* g) {
   if (g->type == 0)
     return usx::ScanGrid::cartesian(g->a0_min, g->a0_max, g->n0, g->a1_min, g->a1_max, g->n1);
   return usx::ScanGrid::sector(g->a0_min, g->a0_max, g->n0, g->a1_min, g->a1_max, g->n1,
                                usx::Vec3(g->apex[0], g->apex[1], g->apex[2]));
+}
+// A line subset keeps the full grid's axis values (the reference's ScanGrid is
+// an explicit point list, types.hpp:228-231, so any subset is a valid grid).
+usx::ScanGrid to_grid(const oc_grid* g) {
+  usx::ScanGrid G = full_grid(g);
+  if (g->line_step <= 0) return G;
+  if (g->line_first < 0 || g->line_first >= G.n_axis0)
+    throw std::invalid_argument("oc_grid: line_first outside the grid");
+  std::vector<float> a0;
+  for (int i = g->line_first; i < G.n_axis0; i += g->line_step) a0.push_back(G.axis0[i]);
+  G.axis0 = a0;
+  G.n_axis0 = static_cast<int>(a0.size());
+  G.build_points();
+  G.validate();
+  return G;
 }
 usx::DemodConfig to_demod(const oc_demod* d) {
   usx::DemodConfig c;
@@ -282,12 +297,30 @@ int oc_demodulate(const oc_setup* s, const oc_transmit* txs, int n_tx, const flo
   });
 }
 
+// Minimum variance with a negative range average makes the reference size a
+// vector of 2 r + 1 entries inside its OpenMP region (beamformer.cpp:357),
+// which ends the process; refuse it here instead of calling in.
+static void refuse_unrunnable(const oc_bf* bf) {
+  if (bf->combiner == 2 && bf->mv_range_average < 0)
+    throw std::invalid_argument("BeamformerConfig.mv_range_average must be >= 0");
+}
+
+// The constructor's check (beamformer.hpp:110).
+int oc_validate_bf(const oc_bf* bf) {
+  return guarded([&] { const usx::Beamformer b(to_bf(bf)); (void)b; });
+}
+
 int oc_beamform(const oc_setup* s, const oc_transmit* txs, int n_tx, int is_iq,
                 const float* data, float fdem, const oc_grid* g, const oc_bf* bf,
                 int event_begin, int event_end, float* frame_out) {
   return guarded([&] {
     const usx::ChannelData cd = make_data(s, txs, n_tx, is_iq, data, fdem);
-    const usx::Beamformer b(to_bf(bf));
+    // Beamformer.config assigned after construction (pymodule.cpp:430-431):
+    // validated only by the constructor, so an out-of-range field reaches
+    // beamform and its clamps.
+    refuse_unrunnable(bf);
+    usx::Beamformer b;
+    b.config() = to_bf(bf);
     const usx::Frame f = b.beamform(cd, to_grid(g), event_begin, event_end);
     std::memcpy(static_cast<void*>(frame_out), static_cast<const void*>(f.data.data()), f.data.size() * sizeof(std::complex<float>));
   });
@@ -301,7 +334,10 @@ int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const 
     usx::PipelineConfig pc;
     pc.demod = to_demod(d);
     pc.beamformer = to_bf(bf);
-    usx::ImagingPipeline pipe(to_grid(g), pc);
+    // ImagingPipeline.config assigned after construction (pymodule.cpp:617-619).
+    refuse_unrunnable(bf);
+    usx::ImagingPipeline pipe(to_grid(g), usx::PipelineConfig{});
+    pipe.config() = pc;
     const usx::Frame f = pipe.process_frame(cd);
     std::memcpy(static_cast<void*>(frame_out), static_cast<const void*>(f.data.data()), f.data.size() * sizeof(std::complex<float>));
     if (stats_out) {

@@ -193,12 +193,26 @@ void validate_setup(const oc_setup& s, const oc_transmit* txs, int n_tx) {
   }
 }
 
+// BeamformerConfig::validate (beamformer.cpp:218-225). The reference runs it
+// when a Beamformer is constructed (beamformer.hpp:110), never inside
+// beamform (beamformer.cpp:297-299): a config assigned after construction is
+// beamformed with the clamps of geometry.cpp:127 and beamformer.cpp:147, 187.
+// The beamform entries below follow that; oc_validate_bf is the constructor's
+// check on its own.
 void validate_bf(const oc_bf& b) {
   if (b.f_number <= 0) throw Invalid("BeamformerConfig.f_number must be > 0");
   if (b.mv_subarray < 2) throw Invalid("BeamformerConfig.mv_subarray must be >= 2");
   if (b.mv_diagonal_loading < 0)
     throw Invalid("BeamformerConfig.mv_diagonal_loading must be >= 0");
   if (b.mv_range_average < 0) throw Invalid("BeamformerConfig.mv_range_average must be >= 0");
+}
+
+// What beamform itself cannot run: minimum variance with a negative range
+// average sizes a scratch vector of 2 r + 1 entries (beamformer.cpp:357),
+// which fails for r < 0; every other out-of-range field has a clamp.
+void check_bf_runnable(const oc_bf& b) {
+  if (b.combiner == 2 && b.mv_range_average < 0)
+    throw Invalid("BeamformerConfig.mv_range_average must be >= 0");
 }
 
 // ---- grid (types.cpp:66-115) ------------------------------------------------
@@ -208,6 +222,24 @@ struct Grid {
   std::vector<float> axis0, axis1;
   std::vector<P3> pts;
 };
+
+// Keep lines line_first, line_first + line_step, ... (line_step 0: all) with
+// their axis values and points as the full grid has them.
+Grid subset_lines(Grid G, const oc_grid& g) {
+  if (g.line_step <= 0) return G;
+  if (g.line_first < 0 || g.line_first >= G.n0) throw Invalid("oc_grid: line_first outside the grid");
+  Grid S;
+  S.type = G.type;
+  S.n1 = G.n1;
+  S.axis1 = G.axis1;
+  for (int i = g.line_first; i < G.n0; i += g.line_step) {
+    S.axis0.push_back(G.axis0[i]);
+    S.pts.insert(S.pts.end(), G.pts.begin() + static_cast<std::ptrdiff_t>(i) * G.n1,
+                 G.pts.begin() + static_cast<std::ptrdiff_t>(i + 1) * G.n1);
+  }
+  S.n0 = static_cast<int>(S.axis0.size());
+  return S;
+}
 
 Grid build_grid(const oc_grid& g) {
   Grid G;
@@ -232,7 +264,7 @@ Grid build_grid(const oc_grid& g) {
         p = {g.apex[0] + r * std::sin(th), 0.0f, g.apex[2] + r * std::cos(th)};
       }
     }
-  return G;
+  return subset_lines(G, g);
 }
 
 // ---- demodulation (demodulate.cpp:13-121) ----------------------------------
@@ -873,6 +905,10 @@ int oc_demodulate(const oc_setup* s, const oc_transmit* txs, int n_tx, const flo
   });
 }
 
+int oc_validate_bf(const oc_bf* bf) {
+  return guarded([&] { validate_bf(*bf); });
+}
+
 // Port-only extension: beamform with optional per-pixel outputs.
 // mv_fp64: 1 runs the MV combine in double (same FP32 channel data).
 int oc_beamform_ex(const oc_setup* s, const oc_transmit* txs, int n_tx, int is_iq,
@@ -881,7 +917,7 @@ int oc_beamform_ex(const oc_setup* s, const oc_transmit* txs, int n_tx, int is_i
                    float* weights_out, float* ych_out, int* info_out) {
   return guarded([&] {
     check_data(*s, txs, n_tx, is_iq != 0, fdem);
-    validate_bf(*bf);
+    check_bf_runnable(*bf);
     const Grid G = build_grid(*g);
     BfExtra ex;
     ex.mv_fp64 = mv_fp64;
@@ -942,7 +978,7 @@ int oc_process_frame(const oc_setup* s, const oc_transmit* txs, int n_tx, const 
   using clk = std::chrono::steady_clock;
   return guarded([&] {
     const auto t_start = clk::now();
-    validate_bf(*bf);
+    check_bf_runnable(*bf);
     const Grid G = build_grid(*g);
     check_data(*s, txs, n_tx, false, 0.0f);
     const auto t_d = clk::now();

@@ -226,18 +226,53 @@ def test_point_target_physics(usx_mod):
     assert widths["MV"] < 0.95 * widths["DAS"]
 
 
-def test_invalid_configs_raise(usx_mod, small):
-    bfc = usx_mod.BeamformerConfig()
-    bfc.f_number = -1.0
-    with pytest.raises(ValueError):
-        usx_mod.Beamformer(bfc)
+@pytest.mark.parametrize("field,value,combiner", [
+    ("f_number", -1.0, "DAS"), ("f_number", 0.0, "DMAS"), ("f_number", -2.0, "MV"),
+    ("mv_subarray", 1, "MV"), ("mv_subarray", -3, "MV"), ("mv_diagonal_loading", -1.0, "MV"),
+    ("mv_range_average", -1, "DAS")])
+def test_config_assigned_after_construction_matches_oracle(usx_mod, port, O, small, small_iq,
+                                                           field, value, combiner):
+    """The reference validates BeamformerConfig when a Beamformer is built
+    (beamformer.hpp:110) and never in beamform (beamformer.cpp:297-299): a
+    config assigned afterwards (pymodule.cpp:430-431, 617-619) runs with the
+    clamps of geometry.cpp:127 and beamformer.cpp:147, 187. The oracle follows
+    the compiled reference bit for bit there (tests/test_oracle.py)."""
+    bad = bf_config(usx_mod, combiner, subarray=8)
+    setattr(bad, field, value)
+    with pytest.raises(ValueError, match=field):
+        usx_mod.Beamformer(bad)
+    pc = usx_mod.PipelineConfig()
+    pc.demod.decimation = 4
+    pc.beamformer = bf_config(usx_mod, combiner, subarray=8)
+    pipe = usx_mod.ImagingPipeline(small.grid, pc)
+    cfg = pipe.config
+    cfg.beamformer = bad
+    pipe.config = cfg
+    got = np.asarray(pipe.process_frame(small.raw).data)
+    ref = _oracle_frame(port, O, small, bad)
+    tol = TOL_MV if combiner == "MV" else TOL_DAS
+    assert np.any(ref != 0) and rel_l2(got, ref) < tol, f"{field}={value}: {rel_l2(got, ref):.3e}"
+    bf = usx_mod.Beamformer(bf_config(usx_mod, combiner, subarray=8))
+    bf.config = bad
+    got = np.asarray(bf.beamform(small_iq, small.grid).data)
+    assert rel_l2(got, ref) < tol
+
+
+def test_mv_with_negative_range_average_raises(usx_mod, port, O, small):
+    """Minimum variance with a negative range average is the one mutated
+    config the reference cannot run (its 2 r + 1 entry scratch vector,
+    beamformer.cpp:357): ValueError, as the oracle reports it."""
+    bad = bf_config(usx_mod, "MV", subarray=8)
+    bad.mv_range_average = -1
     pc = usx_mod.PipelineConfig()
     pipe = usx_mod.ImagingPipeline(small.grid, pc)
     cfg = pipe.config
-    cfg.beamformer.mv_subarray = 1
+    cfg.beamformer = bad
     pipe.config = cfg
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError, match="mv_range_average"):
         pipe.process_frame(small.raw)
+    with pytest.raises(ValueError, match="mv_range_average"):
+        _oracle_frame(port, O, small, bad)
 
 
 def test_frame_metadata_and_stats(usx_mod, small):

@@ -99,6 +99,42 @@ def test_port_matches_reference_live(port, ref, O, seed, n_el, n_tx, fnum):
                     assert bitwise(a, r), (dec, comb, cf, comp)
 
 
+# Fields a BeamformerConfig can hold once assigned after construction, which
+# the constructor would reject (beamformer.cpp:218-225) and beamform clamps.
+MUTATED = [("f_number", -1.0, 0), ("f_number", 0.0, 1), ("mv_subarray", 1, 2),
+           ("mv_subarray", -7, 2), ("mv_diagonal_loading", -1.0, 2), ("f_number", -3.0, 2),
+           ("mv_range_average", -1, 0), ("mv_range_average", -2, 1)]
+
+
+def test_port_matches_reference_on_configs_assigned_after_construction(port, ref, O):
+    s = O.Setup(n_elements=24, bandwidth=0.65, sampling_frequency=16e6, n_samples=600)
+    txs = ref.make_transmits(s, O.SimCfg(), 1, np.deg2rad(np.linspace(-8, 8, 3)).astype(np.float32))
+    pos, amp = ref.speckle_box(-0.006, 0.006, 0.004, 0.022, 2e-4, 300, 4, 1.0)
+    rf = ref.simulate(s, O.SimCfg(), txs, pos, amp)
+    g = O.GridSpec(0, -0.004, 0.004, 10, 0.005, 0.02, 30)
+    for field, value, comb in MUTATED:
+        b = O.Bf(**{"combiner": comb, "mv_subarray": 6, field: value})
+        for lib in (port, ref):
+            with pytest.raises(ValueError, match=field):
+                lib.validate_bf(b)
+        a, _ = port.process_frame(s, txs, rf, g, O.Demod(decimation=2), b)
+        r, _ = ref.process_frame(s, txs, rf, g, O.Demod(decimation=2), b)
+        assert bitwise(a, r), (field, value, comb)
+        assert np.any(a != 0)
+        # the clamps: the same frame as the in-range value they clamp to
+        clamp = {"f_number": 1e-3, "mv_subarray": 2, "mv_diagonal_loading": 0.0}.get(field)
+        if clamp is not None:
+            c, _ = port.process_frame(s, txs, rf, g, O.Demod(decimation=2),
+                                      O.Bf(**{"combiner": comb, "mv_subarray": 6, field: clamp}))
+            assert bitwise(a, c), (field, value)
+    # minimum variance with a negative range average cannot run in the reference
+    bad = O.Bf(combiner=2, mv_range_average=-1)
+    for lib in (port, ref):
+        with pytest.raises(ValueError, match="mv_range_average"):
+            lib.process_frame(s, txs, rf, g, O.Demod(decimation=2), bad)
+    assert port.validate_bf(O.Bf()) is None and ref.validate_bf(O.Bf()) is None
+
+
 def test_port_matches_reference_small_functions(port, ref, O):
     for w in range(5):
         for u in np.linspace(-1.2, 1.2, 97, dtype=np.float32):

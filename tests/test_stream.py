@@ -124,3 +124,17 @@ def test_device_runner_graph_equals_eager(usx_mod, O, combiner):
         run.run_graph()
         run.synchronize()
         assert np.array_equal(_bits(run.graph_frame()), _bits(ref))
+    if combiner == "DAS":
+        # a capture on caller buffers frees capture_host's copies: no stale frame
+        rf_host = np.ascontiguousarray(np.asarray(case.raw.rf))
+        rf = usx_mod.DeviceBuffer(rf_host.nbytes)
+        rf.upload(rf_host)
+        shape = [case.grid.n_axis0, case.grid.n_axis1]
+        fr = usx_mod.DeviceBuffer(8 * shape[0] * shape[1])
+        fr.zero()
+        run.capture(rf.ptr, fr.ptr)
+        with pytest.raises(Exception, match="capture_host"):
+            run.graph_frame()
+        run.run_graph()
+        run.synchronize()
+        assert np.array_equal(_bits(fr.download(np.complex64, shape)), _bits(ref))

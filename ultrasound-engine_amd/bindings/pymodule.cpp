@@ -53,6 +53,44 @@ py::array_t<T> copy_vector(const std::vector<T>& v, std::vector<py::ssize_t> sha
   return a;
 }
 
+// Device memory for callers without an allocator of their own (tests and
+// graph-style callers of DeviceRunner): usx_buffer_alloc on the calling
+// thread's context, freed when the object goes; copies are synchronous.
+class DeviceBuffer {
+ public:
+  explicit DeviceBuffer(std::size_t bytes) : ctx_(thread_context().get()), n_(bytes) {
+    check(ctx_, usx_buffer_alloc(ctx_, bytes, &p_), "buffer_alloc");
+  }
+  ~DeviceBuffer() {
+    if (p_) usx_buffer_free(ctx_, p_);
+  }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  std::uintptr_t ptr() const { return reinterpret_cast<std::uintptr_t>(p_); }
+  std::size_t nbytes() const { return n_; }
+  void upload(const py::array& a) {
+    const py::array c = py::array::ensure(a, py::array::c_style);
+    if (static_cast<std::size_t>(c.nbytes()) > n_)
+      throw std::invalid_argument("DeviceBuffer.upload: array larger than the buffer");
+    check(ctx_, usx_buffer_upload(ctx_, p_, c.data(), static_cast<std::size_t>(c.nbytes())),
+          "buffer_upload");
+  }
+  py::array download(const py::object& dtype, const std::vector<py::ssize_t>& shape) {
+    py::array a(py::dtype::from_args(dtype), shape);
+    if (static_cast<std::size_t>(a.nbytes()) > n_)
+      throw std::invalid_argument("DeviceBuffer.download: shape larger than the buffer");
+    check(ctx_, usx_buffer_download(ctx_, a.mutable_data(), p_, static_cast<std::size_t>(a.nbytes())),
+          "buffer_download");
+    return a;
+  }
+  void zero() { check(ctx_, usx_buffer_zero(ctx_, p_, n_), "buffer_zero"); check(ctx_, usx_ctx_synchronize(ctx_), "synchronize"); }
+
+ private:
+  usx_ctx* ctx_;
+  void* p_ = nullptr;
+  std::size_t n_;
+};
+
 // Device-resident runner for benchmarks and graph-style callers: descriptors
 // are built once from a template ChannelData, then frames are beamformed
 // from device pointers (e.g. torch tensors' data_ptr()) on the runner's own
@@ -112,6 +150,16 @@ class DeviceRunner {
     usx_ctx* c = ctx_.get();
     if (graph_) usx_plan_destroy(c, graph_);
     graph_ = nullptr;
+    // A capture on the caller's buffers ends an earlier capture_host: its
+    // device copies go, so graph_frame() cannot return that capture's frame.
+    if (graph_rf_ && reinterpret_cast<void*>(in_ptr) != graph_rf_) {
+      usx_buffer_free(c, graph_rf_);
+      graph_rf_ = nullptr;
+    }
+    if (graph_frame_ && reinterpret_cast<void*>(frame_ptr) != graph_frame_) {
+      usx_buffer_free(c, graph_frame_);
+      graph_frame_ = nullptr;
+    }
     const float* rf = reinterpret_cast<const float*>(in_ptr);
     usx_c32* fr = reinterpret_cast<usx_c32*>(frame_ptr);
     check(c, usx_plan_create(c, 1, &dplan_.desc, &bplan_.desc, &rf, &fr, &graph_), "plan_create");
@@ -1112,6 +1160,14 @@ PYBIND11_MODULE(_usx, m) {
       .def_property_readonly("frame_count", &ImagingPipeline::frame_count)
       .def_property("config", py::overload_cast<>(&ImagingPipeline::config, py::const_),
                     [](ImagingPipeline& p, const PipelineConfig& c) { p.config() = c; });
+
+  py::class_<DeviceBuffer>(m, "DeviceBuffer")
+      .def(py::init<std::size_t>(), py::arg("nbytes"))
+      .def_property_readonly("ptr", &DeviceBuffer::ptr)
+      .def_property_readonly("nbytes", &DeviceBuffer::nbytes)
+      .def("upload", &DeviceBuffer::upload, py::arg("array"))
+      .def("download", &DeviceBuffer::download, py::arg("dtype"), py::arg("shape"))
+      .def("zero", &DeviceBuffer::zero);
 
   py::class_<DeviceRunner>(m, "DeviceRunner")
       .def(py::init<const ScanGrid&, const PipelineConfig&, const ChannelData&>(), py::arg("grid"),

@@ -60,7 +60,6 @@ Frame Beamformer::beamform(const ChannelData& data, const ScanGrid& grid, int ev
                            int event_end) const {
   data.validate();
   grid.validate();
-  config_.validate();
   const int n_ev = data.n_events();
   if (event_end < 0) event_end = n_ev;
   event_begin = std::clamp(event_begin, 0, n_ev);

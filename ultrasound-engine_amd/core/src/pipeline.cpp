@@ -49,7 +49,6 @@ Frame ImagingPipeline::process_frame(const ChannelData& data) {
   const auto t_start = Clock::now();
   stats_ = PipelineStats{};
   data.validate();
-  config_.beamformer.validate();
   usx_ctx* ctx = gpu_->get();
   BeamformerConfig bcfg = config_.beamformer;
   if (bcfg.combiner == Combiner::Learned && !has_apod_) bcfg.combiner = Combiner::MV;
@@ -112,7 +111,6 @@ std::vector<real> ImagingPipeline::process_display(const ChannelData& data) {
   const auto t_start = Clock::now();
   stats_ = PipelineStats{};
   data.validate();
-  config_.beamformer.validate();
   if (data.rf.empty()) throw std::invalid_argument("demodulate: input has no RF samples");
   usx_ctx* ctx = gpu_->get();
   BeamformerConfig bcfg = config_.beamformer;

@@ -46,14 +46,17 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // Tile decode with XCD-aware remapping of the workgroup index: one tile is
-// 64 depths of one lateral line.
+// 64 depths of one lateral line. Workgroups are dealt to the XCDs round
+// robin (workgroup b runs on XCD b mod n_xcd), so tile (b mod X) * per + b / X
+// gives each XCD, and its own L2, a contiguous range of lines.
 __device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
-  const int per = (nb + 7) / 8;
-  int t = (b % 8) * per + b / 8;
+  const int X = A.n_xcd;
+  const int per = (nb + X - 1) / X;
+  int t = (b % X) * per + b / X;
   // Blocks beyond an uneven last XCD slice fall back to the identity map.
-  if (nb % 8 != 0) t = b;
+  if (nb % X != 0) t = b;
   const int nj = (A.g.n1 + 63) / 64;
   const int ti = t / nj, tj = t - ti * nj;
   *i0 = ti;

@@ -49,6 +49,8 @@ struct Cached {
 
 struct usx_ctx {
   int device = -1;
+  int n_cu = 256;   // compute units (hipDeviceAttributeMultiprocessorCount)
+  int n_xcd = 8;    // accelerator complex dies, each with its own L2 (hipDeviceAttributeNumberOfXccs)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t user_ev[8] = {};
@@ -155,6 +157,10 @@ void check_demod(usx_ctx* c, const usx_demod_desc* d) {
     fail(c, USX_ERR_INVALID, "demodulate: taps and rotator tables are required");
 }
 
+// Subarray length as minimum variance uses it: clamp(L, 2, ...) lifts
+// anything below 2 (beamformer.cpp:147).
+int mv_sub(const usx_bf_desc* b) { return std::max(2, b->mv_subarray); }
+
 void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (!b) fail(c, USX_ERR_INVALID, "usx_beamform: null descriptor");
   if (b->n_events <= 0) fail(c, USX_ERR_INVALID, "ChannelData has no transmit events");
@@ -170,11 +176,15 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->grid_type == 1 && (!b->axis0_sin || !b->axis0_cos))
     fail(c, USX_ERR_INVALID, "beamform: sector grids need the sin/cos tables");
   if (b->grid_type != 0 && b->grid_type != 1) fail(c, USX_ERR_INVALID, "beamform: unknown grid type");
-  if (!(b->f_number > 0)) fail(c, USX_ERR_INVALID, "BeamformerConfig.f_number must be > 0");
-  if (b->mv_subarray < 2) fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_subarray must be >= 2");
-  if (b->mv_diagonal_loading < 0)
-    fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_diagonal_loading must be >= 0");
-  if (b->mv_range_average < 0)
+  // BeamformerConfig is validated when a Beamformer is constructed
+  // (beamformer.hpp:110), not per call: a config assigned afterwards
+  // (pymodule.cpp:430-431, 617-618) runs with the reference's clamps instead,
+  // f_number through max(f, 1e-3) (geometry.cpp:127), the subarray through
+  // clamp(L, 2, ...) (beamformer.cpp:147) and the loading through
+  // max(delta tr / L, 1e-20) (beamformer.cpp:187). A negative range average
+  // has no clamp there (its scratch vector of 2 r + 1 entries cannot be
+  // built, beamformer.cpp:357), so it stays an error for minimum variance.
+  if (b->combiner == 2 && b->mv_range_average < 0)
     fail(c, USX_ERR_INVALID, "BeamformerConfig.mv_range_average must be >= 0");
   if (b->rx_window < 0 || b->rx_window > 4) fail(c, USX_ERR_INVALID, "unknown window");
   if (b->combiner < 0 || b->combiner > 3) fail(c, USX_ERR_INVALID, "unknown combiner");
@@ -188,13 +198,13 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->tile_slots < 0 || b->tile_slots > 16 || b->tile_mv_occupancy < 0 || b->tile_mv_occupancy > 32)
     fail(c, USX_ERR_INVALID, "beamform: tile_slots must be in 0..16 and tile_mv_occupancy in 0..32");
   if (b->combiner == 2) {
-    const int pmax = usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray);
+    const int pmax = usxd::mv_max_bucket_p(b->n_channels, mv_sub(b));
     if (pmax > usxd::kMvMaxP)
       fail(c, USX_ERR_INVALID,
            "MV: subarray length up to %d with %d channels exceeds the supported %d",
-           std::min(b->mv_subarray, b->n_channels / 2), b->n_channels, usxd::kMvMaxP);
+           std::min(mv_sub(b), b->n_channels / 2), b->n_channels, usxd::kMvMaxP);
     const size_t lds =
-        usxd::mv_smem_bytes(b->n_channels, b->mv_range_average, b->mv_subarray, pmax, b->mv_refine != 0);
+        usxd::mv_smem_bytes(b->n_channels, b->mv_range_average, mv_sub(b), pmax, b->mv_refine != 0);
     if (lds > 160 * 1024)
       fail(c, USX_ERR_INVALID,
            "MV: %d channels with range average %d need %zu B of LDS per pixel (limit 163840)",
@@ -251,7 +261,7 @@ struct BfTables {
   const float* c0 = nullptr;
 };
 
-BfArgs args_of(const usx_bf_desc* b, const BfTables& t) {
+BfArgs args_of(const usx_ctx* c, const usx_bf_desc* b, const BfTables& t) {
   BfArgs A{};
   A.n_ch = b->n_channels;
   A.n_s = b->n_samples;
@@ -270,6 +280,7 @@ BfArgs args_of(const usx_bf_desc* b, const BfTables& t) {
   A.rx_window = b->rx_window;
   A.tukey_alpha = b->tukey_alpha;
   A.transmit_weighting = b->transmit_weighting;
+  A.n_xcd = c->n_xcd;
   A.ex = t.ex;
   A.tx = t.tx;
   A.g.type = b->grid_type;
@@ -294,7 +305,7 @@ BfArgs make_args(usx_ctx* c, const usx_bf_desc* b) {
     t.s0 = static_cast<const float*>(upload(c, c->s0, b->axis0_sin, sizeof(float) * b->n_axis0));
     t.c0 = static_cast<const float*>(upload(c, c->c0, b->axis0_cos, sizeof(float) * b->n_axis0));
   }
-  return args_of(b, t);
+  return args_of(c, b, t);
 }
 
 // Launch-shape tunables of one beamforming call (what the autotuner picks):
@@ -352,7 +363,7 @@ ScratchSize scratch_size(const usx_bf_desc* b, const Tile& T) {
   if (incoh) z.acc = n_pix * sizeof(float);
   if (comb == 2) {
     z.counts = 16 * sizeof(int);
-    z.lists = static_cast<size_t>(usxd::mv_max_bucket_p(b->n_channels, b->mv_subarray) / 16) * n_pix * sizeof(int);
+    z.lists = static_cast<size_t>(usxd::mv_max_bucket_p(b->n_channels, mv_sub(b)) / 16) * n_pix * sizeof(int);
   }
   return z;
 }
@@ -396,13 +407,14 @@ void enqueue_beamform(usx_ctx* c, const usx_bf_desc* b, const BfArgs& A, const S
   M.n0 = b->n_axis0;
   M.n1 = b->n_axis1;
   M.n_ch = b->n_channels;
-  M.subarray = b->mv_subarray;
+  M.subarray = mv_sub(b);
   M.loading = b->mv_diagonal_loading;
   M.range_avg = b->mv_range_average;
   M.cf = cf;
   M.precision = b->mv_precision;
   M.refine = b->mv_refine;
   M.occupancy = T.mv_occupancy;
+  M.n_cu = c->n_cu;
   if (comb == 3 && !c->apod_loaded)
     fail(c, USX_ERR_INVALID, "learned apodization: no network loaded (usx_apod_set_params)");
   auto mv = [&](float* acc) {
@@ -567,6 +579,11 @@ usx_status usx_ctx_create(int device, usx_ctx** out) {
   (void)registered;
   usx_ctx* c = new usx_ctx();
   c->device = device;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
+    c->n_cu = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && v > 0)
+    c->n_xcd = v;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return USX_ERR_RUNTIME;
@@ -1165,7 +1182,7 @@ void plan_tables(usx_ctx* c, usx_plan* p, int f) {
   bt.ax1 = static_cast<const float*>(at(5));
   bt.s0 = static_cast<const float*>(at(6));
   bt.c0 = static_cast<const float*>(at(7));
-  p->args[f] = args_of(&b, bt);
+  p->args[f] = args_of(c, &b, bt);
 }
 
 }  // namespace

@@ -27,6 +27,7 @@ struct BfArgs {
   int rx_window;
   float tukey_alpha;
   int transmit_weighting;
+  int n_xcd;                 // XCDs of the device: the tile remap gives each a contiguous range
 };
 
 // demod.hip
@@ -62,6 +63,7 @@ struct MvArgs {
   float2* weights;
   int* status;
   int occupancy;   // persistent workgroups per CU per bucket (0: as many as LDS allows, <= 32)
+  int n_cu;        // compute units of the device (sizes the persistent grids)
 };
 // counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,

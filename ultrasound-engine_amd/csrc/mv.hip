@@ -1440,7 +1440,7 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
       return 4 * std::max(1, std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8)));                    \
     }();                                                                                         \
     const int per_cu = std::max(1, std::min(std::min(cap, lds_cap), vgpr_cap));                  \
-    const int grid = std::min(n_pix, 256 * per_cu);                                              \
+    const int grid = std::min(n_pix, M.n_cu * per_cu);                                           \
     hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(64), smem, st, M, Y, mask, frame, acc, counts,   \
                        lists __VA_OPT__(, ) __VA_ARGS__);                                        \
   } while (0)

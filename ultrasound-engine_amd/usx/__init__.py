@@ -30,6 +30,7 @@ from ._usx import (  # noqa: F401
     Combiner,
     Compounding,
     DemodConfig,
+    DeviceBuffer,
     DeviceRunner,
     DisplacementMap,
     DopplerConfig,
