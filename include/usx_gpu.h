@@ -195,6 +195,26 @@ usx_status usx_process_frame(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf
                              const float* rf, usx_c32* frame, int memory,
                              usx_stage_times* times);
 
+/* ---- allocation-free frame (no reference counterpart) -------------------
+ *
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The same
+ * work as usx_process_frame with memory = USX_DEVICE (RF in device memory ->
+ * demodulate -> beamform -> frame in device memory, the same kernels and the
+ * same results), but no call allocates: the IQ, the descriptor tables and the
+ * beamforming scratch (channel vectors, active masks, minimum-variance work
+ * lists) are laid out in one fixed array of USX_NOALLOC_SCRATCH_BYTES per GPU,
+ * which the library holds from its first context on that GPU until the
+ * process ends. A frame that needs more (the minimum-variance channel vectors
+ * of large grids, for one) returns USX_ERR_INVALID with the bytes it needs.
+ * Asynchronous on the context stream; the host descriptor arrays are copied
+ * before the call returns. Every context on one GPU shares that GPU's array,
+ * which keeps the last call's contents: calls that overlap on one GPU (from
+ * two contexts, or enqueued on two streams) race. */
+#define USX_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+usx_status usx_process_frame_noalloc(usx_ctx* ctx, const usx_demod_desc* d, const usx_bf_desc* b,
+                                     const float* rf, usx_c32* frame);
+
 /* ---- display tail (pipeline.cpp:48-79, postprocess.cpp:12-117) --------- */
 
 typedef struct {

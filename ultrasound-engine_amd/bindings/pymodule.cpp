@@ -142,6 +142,16 @@ class DeviceRunner {
             "beamform");
     }
   }
+  // run() through the allocation-free entry point (RF runners only): the
+  // scratch comes from the library's fixed per-GPU array, not the context.
+  void run_noalloc(std::uintptr_t in_ptr, std::uintptr_t frame_ptr) {
+    if (!is_rf_) throw std::invalid_argument("run_noalloc: the runner's template must be RF");
+    usx_ctx* c = ctx_.get();
+    check(c, usx_process_frame_noalloc(c, &dplan_.desc, &bplan_.desc,
+                                       reinterpret_cast<const float*>(in_ptr),
+                                       reinterpret_cast<usx_c32*>(frame_ptr)),
+          "process_frame_noalloc");
+  }
   // The frame of run() with the RF and frame pointers bound now, recorded
   // into a HIP graph (a one-frame usx_plan: one eager pass, then the
   // capture); run_graph() replays it on the runner's stream.
@@ -1173,6 +1183,8 @@ PYBIND11_MODULE(_usx, m) {
       .def(py::init<const ScanGrid&, const PipelineConfig&, const ChannelData&>(), py::arg("grid"),
            py::arg("config"), py::arg("like"))
       .def("run", &DeviceRunner::run, py::arg("input_ptr"), py::arg("frame_ptr"),
+           py::call_guard<py::gil_scoped_release>())
+      .def("run_noalloc", &DeviceRunner::run_noalloc, py::arg("input_ptr"), py::arg("frame_ptr"),
            py::call_guard<py::gil_scoped_release>())
       .def("synchronize", &DeviceRunner::synchronize, py::call_guard<py::gil_scoped_release>())
       .def("capture", &DeviceRunner::capture, py::arg("input_ptr"), py::arg("frame_ptr"),

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -51,6 +52,8 @@ struct usx_ctx {
   int device = -1;
   int n_cu = 256;   // compute units (hipDeviceAttributeMultiprocessorCount)
   int n_xcd = 8;    // accelerator complex dies, each with its own L2 (hipDeviceAttributeNumberOfXccs)
+  void* scratch = nullptr;        // this device's allocation-free scratch (see bind_scratch)
+  hipError_t scratch_err = hipSuccess;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t user_ev[8] = {};
@@ -79,6 +82,35 @@ struct usx_ctx {
   Cached dop_reject;
   DevBuf dop_in, dop_in2, dop_out, dop_lag, dop_f0, dop_f1, dop_f2, dop_valid, dop_pmax;
 };
+
+// ---- scratch of the allocation-free entry point -----------------------------
+// One fixed array per GPU (each device that loads this code object holds its
+// own copy until the process ends). In host code `scratch` is only a handle
+// for hipGetSymbolAddress, never a device address: every use goes through the
+// per-device pointer bind_scratch returns.
+static __device__ __align__(256) unsigned char scratch[USX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current:
+// hipGetSymbolAddress once per device, the result kept for every later context.
+hipError_t bind_scratch(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
 
 namespace {
 // Set once the process starts exiting; later destroys skip HIP calls, which
@@ -588,6 +620,7 @@ usx_status usx_ctx_create(int device, usx_ctx** out) {
     delete c;
     return USX_ERR_RUNTIME;
   }
+  c->scratch_err = bind_scratch(device, &c->scratch);
   for (auto& e : c->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       usx_ctx_destroy(c);
@@ -785,6 +818,70 @@ usx_status usx_process_frame(usx_ctx* c, const usx_demod_desc* d, const usx_bf_d
       t->beamform_ms = bb;
       t->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     }
+  });
+}
+
+usx_status usx_process_frame_noalloc(usx_ctx* c, const usx_demod_desc* d, const usx_bf_desc* b,
+                                     const float* rf, usx_c32* frame) {
+  return guarded(c, [&] {
+    check_demod(c, d);
+    check_bf(c, b);
+    if (!rf || !frame) fail(c, USX_ERR_INVALID, "usx_process_frame_noalloc: null data pointer");
+    if (!b->is_iq) fail(c, USX_ERR_INVALID, "usx_process_frame_noalloc: beamform descriptor must describe IQ");
+    if (b->n_events * b->n_channels != d->n_traces || b->n_samples != d->n_out)
+      fail(c, USX_ERR_INVALID, "usx_process_frame_noalloc: demodulate and beamform shapes disagree");
+    if (!c->scratch)
+      fail(c, USX_ERR_RUNTIME, "usx_process_frame_noalloc: scratch array unavailable (hipGetSymbolAddress: %s)",
+           hipGetErrorString(c->scratch_err));
+    // Lay out every buffer of the call in the scratch array, 256-byte aligned:
+    // descriptor tables, IQ, then what the beamforming launch sequence needs.
+    const Tile T = tile_of(b);
+    const ScratchSize z = scratch_size(b, T);
+    const bool sector = b->grid_type == 1;
+    const size_t sizes[] = {sizeof(float) * d->n_taps, sizeof(usx_c32) * d->n_in,
+                            sizeof(float) * b->n_channels, sizeof(usx_tx_desc) * b->n_events,
+                            sizeof(float) * b->n_axis0, sizeof(float) * b->n_axis1,
+                            sector ? sizeof(float) * b->n_axis0 : 0, sector ? sizeof(float) * b->n_axis0 : 0,
+                            sizeof(usx_c32) * static_cast<size_t>(d->n_traces) * d->n_out,
+                            z.y, z.mask, z.acc, z.counts, z.lists};
+    constexpr int kParts = sizeof(sizes) / sizeof(sizes[0]);
+    size_t off[kParts], used = 0;
+    for (int i = 0; i < kParts; ++i) {
+      off[i] = used;
+      used += (sizes[i] + 255) / 256 * 256;
+    }
+    if (used > USX_NOALLOC_SCRATCH_BYTES)
+      fail(c, USX_ERR_INVALID,
+           "usx_process_frame_noalloc: this frame needs %zu B of scratch; the allocation-free entry "
+           "point holds at most %zu B (use usx_process_frame)",
+           used, static_cast<size_t>(USX_NOALLOC_SCRATCH_BYTES));
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    auto at = [&](int i) -> void* { return sizes[i] ? base + off[i] : nullptr; };
+    const void* host[] = {d->taps, d->rotator, b->element_x, b->tx, b->axis0, b->axis1,
+                          b->axis0_sin, b->axis0_cos};
+    for (int i = 0; i < 8; ++i)
+      if (sizes[i])
+        hip_check(c, hipMemcpyAsync(at(i), host[i], sizes[i], hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync(descriptor)");
+    BfTables t;
+    t.ex = static_cast<const float*>(at(2));
+    t.tx = static_cast<const usx_tx_desc*>(at(3));
+    t.ax0 = static_cast<const float*>(at(4));
+    t.ax1 = static_cast<const float*>(at(5));
+    t.s0 = static_cast<const float*>(at(6));
+    t.c0 = static_cast<const float*>(at(7));
+    const BfArgs A = args_of(c, b, t);
+    float2* d_iq = static_cast<float2*>(at(8));
+    Scratch S;
+    S.Y = static_cast<float2*>(at(9));
+    S.mask = static_cast<uint32_t*>(at(10));
+    S.acc = static_cast<float*>(at(11));
+    S.counts = static_cast<int*>(at(12));
+    S.lists = static_cast<int*>(at(13));
+    enqueue_demod(c, d, static_cast<const float*>(at(0)), static_cast<const float2*>(at(1)), rf, d_iq);
+    check_launch(c, "demodulate launch");
+    enqueue_beamform(c, b, A, S, T, d_iq, reinterpret_cast<float2*>(frame), nullptr, nullptr);
+    check_launch(c, "beamform launch");
   });
 }
 
