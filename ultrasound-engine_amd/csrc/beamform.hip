@@ -28,6 +28,8 @@
 #include <climits>
 
 #include "device_math.hpp"
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace usxd {
@@ -45,22 +47,41 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Tile decode with XCD-aware remapping of the workgroup index: one tile is
-// 64 depths of one lateral line. Workgroups are dealt to the XCDs round
-// robin (workgroup b runs on XCD b mod n_xcd), so tile (b mod X) * per + b / X
-// gives each XCD, and its own L2, a contiguous range of lines.
-__device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
+// A tile is 4 neighbouring lateral lines x 16 consecutive depths, one pixel
+// per lane (lane = 16 * line offset + depth offset). Against 64 depths of one
+// line it narrows the union of the 64 pixels' receive apertures, which the
+// wavefront's channel loop covers: the aperture's half width grows with depth
+// (z / 2F#), and 16 depths spread it a quarter as much, while four lines
+// shift it by under two elements.
+constexpr int kTileLines = 4, kTileDepths = 16;
+
+__device__ __forceinline__ int tiles_per_line_block(const BfArgs& A) {
+  return (A.g.n1 + kTileDepths - 1) / kTileDepths;
+}
+
+// Pixel (line, depth) of lane r of tile t.
+__device__ __forceinline__ void tile_pixel(const BfArgs& A, int t, int r, int* i0, int* j) {
+  const int nj = tiles_per_line_block(A);
+  const int ti = t / nj, tj = t - ti * nj;
+  *i0 = ti * kTileLines + (r >> 4);
+  *j = tj * kTileDepths + (r & 15);
+}
+
+// This workgroup's tile, with XCD-aware remapping of the workgroup index.
+// Workgroups are dealt to the XCDs round robin (workgroup b runs on XCD
+// b mod n_xcd), so tile (b mod X) * per + b / X gives each XCD, and its own
+// L2, a contiguous range of line blocks.
+__device__ __forceinline__ int tile_index(const BfArgs& A) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int X = A.n_xcd;
   const int per = (nb + X - 1) / X;
-  int t = (b % X) * per + b / X;
   // Blocks beyond an uneven last XCD slice fall back to the identity map.
-  if (nb % X != 0) t = b;
-  const int nj = (A.g.n1 + 63) / 64;
-  const int ti = t / nj, tj = t - ti * nj;
-  *i0 = ti;
-  *j = tj * 64 + static_cast<int>(threadIdx.x & 63);
+  return (nb % X != 0) ? b : (b % X) * per + b / X;
+}
+
+__device__ __forceinline__ void tile_of(const BfArgs& A, int* i0, int* j) {
+  tile_pixel(A, tile_index(A), static_cast<int>(threadIdx.x & 63), i0, j);
 }
 
 struct TxSlot {
@@ -72,10 +93,10 @@ struct TxSlot {
 __device__ __forceinline__ TxSlot tx_slot(const BfArgs& A, int ev, float w, const Pt& p) {
   TxSlot s;
   s.ev = ev;
-  s.tau = tx_time(A.tx[ev], p, A.c);
+  s.tau = tx_time(A.tx[ev], p, A.c, A.inv_c);
   if (A.fdem > 0.0f) {
     float sn, cs;
-    sincosf(A.w0 * s.tau, &sn, &cs);
+    rot_of(A.w0 * s.tau, &cs, &sn);
     s.tr = w * cs;
     s.ti = w * sn;
   } else {
@@ -85,9 +106,10 @@ __device__ __forceinline__ TxSlot tx_slot(const BfArgs& A, int ev, float w, cons
   return s;
 }
 
-// Illumination weight of event ev for pixel p, or of the Select winner.
+// Illumination weight of event ev for pixel p (coherent and incoherent
+// compounding: the weight only scales the event's contribution).
 __device__ __forceinline__ float tx_w(const BfArgs& A, int ev, const Pt& p) {
-  return A.transmit_weighting ? tx_weight(A.tx[ev], p, A.lambda) : 1.0f;
+  return A.transmit_weighting ? tx_weight<false>(A.tx[ev], p, A.lambda) : 1.0f;
 }
 
 // Best-illuminating event (Select compounding, beamformer.cpp:394-408):
@@ -96,7 +118,7 @@ __device__ __forceinline__ int select_event(const BfArgs& A, const Pt& p, float*
   int best = -1;
   float bw = 0.0f;
   for (int e = A.ev_begin; e < A.ev_end; ++e) {
-    const float w = tx_weight(A.tx[e], p, A.lambda);
+    const float w = tx_weight<true>(A.tx[e], p, A.lambda);
     if (w > bw) {
       bw = w;
       best = e;
@@ -133,10 +155,10 @@ __device__ __forceinline__ float2 sample(const void* data, size_t base, int n, f
 
 // The same sample from a trace addressed as a buffer resource whose extent is
 // exactly the trace: a load outside [0, n) returns 0 instead of touching
-// memory, so the index needs no clamping into the trace, the second sample is
-// the first's offset + one sample (immediate offset), and the address is a
-// 32-bit lane offset from a scalar base. floor(idx) == float(k) wherever the
-// sample is used (0 <= idx < n - 1 < 2^24), so the fraction is idx - floor(idx).
+// memory, so an index outside the trace needs no clamping, both samples come
+// in one load (16 bytes of IQ, 8 of RF), and the address is a 32-bit lane
+// offset from a scalar base. The interpolation is a + f (b - a), the
+// reference's a (1 - f) + b f (beamformer.cpp:27-39) to an ulp or so.
 template <bool IQ>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t trace_rsrc(const void* data, size_t base, int n) {
   constexpr int kBytes = IQ ? 8 : 4;
@@ -144,26 +166,53 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t trace_rsrc(const void* data, s
   // dword 3 of a raw buffer descriptor on gfx9-family parts (32-bit data, no swizzle)
   return __builtin_amdgcn_make_buffer_rsrc(p, 0, n * kBytes, 0x00020000);
 }
-template <bool IQ>
+template <bool IQ, bool CHK = true>
 __device__ __forceinline__ float2 sample_rsrc(__amdgpu_buffer_rsrc_t t, float nm1, float idx) {
   typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-  const bool ok = idx >= 0.0f && idx < nm1;
-  const float fl = floorf(idx);
-  const int k = ok ? static_cast<int>(fl) : 0;
-  const float f = idx - fl;
-  const float g = 1.0f - f;
-  float2 r;
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  // Outside [0, n - 1) (and for NaN) the index moves to 2^24: its byte
+  // offset lies past every trace, so both samples read 0, and its fraction
+  // is 0, so the result is the reference's 0 without a select on the value.
+  // !CHK: the caller has shown that idx lies inside (gathers_inside).
+  const bool ok = !CHK || (idx >= 0.0f && idx < nm1);
+  const float id = ok ? idx : 16777216.0f;
+  const int k = static_cast<int>(id);             // == floor(id): id >= 0
+  const float f = __builtin_amdgcn_fractf(id);    // id - floor(id), exact
   if (IQ) {
-    const u2 a = __builtin_amdgcn_raw_buffer_load_b64(t, k * 8, 0, 0);
-    const u2 b = __builtin_amdgcn_raw_buffer_load_b64(t, k * 8 + 8, 0, 0);
-    r = make_float2(fmaf(__uint_as_float(b.x), f, __uint_as_float(a.x) * g),
-                    fmaf(__uint_as_float(b.y), f, __uint_as_float(a.y) * g));
+    const u4 ab = __builtin_amdgcn_raw_buffer_load_b128(t, k * 8, 0, 0);
+    const float ax = __uint_as_float(ab.x), ay = __uint_as_float(ab.y);
+    return make_float2(fmaf(f, __uint_as_float(ab.z) - ax, ax), fmaf(f, __uint_as_float(ab.w) - ay, ay));
   } else {
-    const float a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t, k * 4, 0, 0));
-    const float b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t, k * 4 + 4, 0, 0));
-    r = make_float2(fmaf(b, f, a * g), 0.0f);
+    const u2 ab = __builtin_amdgcn_raw_buffer_load_b64(t, k * 4, 0, 0);
+    const float a = __uint_as_float(ab.x);
+    return make_float2(fmaf(f, __uint_as_float(ab.y) - a, a), 0.0f);
   }
-  return ok ? r : make_float2(0.0f, 0.0f);
+}
+
+// A pixel's receive aperture: d = max(half aperture, 1e-9) and r = 1/d, once
+// per pixel (dynamic_aperture_weight, geometry.cpp:123-130); none for
+// pixels at or above the face.
+struct RxAperture {
+  float d, r;
+  bool any;
+};
+__device__ __forceinline__ RxAperture rx_aperture(const BfArgs& A, const Pt& p, bool valid) {
+  RxAperture ap;
+  const float depth = p.z - 0.0f;
+  ap.any = valid && depth > 0.0f;
+  const float half = 0.5f * depth / smax(A.f_number, 1e-3f);
+  ap.d = smax(half, 1e-9f);
+  ap.r = 1.0f / ap.d;
+  return ap;
+}
+// Receive weight of channel ch (*on: the reference's weight is non-zero).
+__device__ __forceinline__ float rx_apod(const BfArgs& A, const RxAperture& ap, int ch, const Pt& p,
+                                        bool* on) {
+  if (!ap.any) {
+    *on = false;
+    return 0.0f;
+  }
+  return rx_weight(A.rx_window, A.tukey_alpha, A.ex[ch] - p.x, ap.d, ap.r, A.cos1_lo, A.cos1_hi, on);
 }
 
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
@@ -220,18 +269,51 @@ __device__ __forceinline__ int tile_slots(const BfArgs& A, const Pt& p, bool val
   return nsl;
 }
 
+// Whether every gather of this wavefront -- its pixels, their live slots and
+// the channel range lo..hi -- reads strictly inside the traces, by a margin of
+// two samples: then its channel loop needs no bounds test per gather. The
+// receive time of a pixel lies between depth / c (an element on the face is
+// at least that far) and the time to the farther end of lo..hi (the elements
+// lie along x, evenly spaced, so distance grows towards the ends). Every
+// pixel must also be deeper than 1e-12 m, which keeps the squared distances
+// normal for rx_time_normal. Wavefront-uniform.
+template <int NE>
+__device__ __forceinline__ bool gathers_inside(const BfArgs& A, const Pt& p, bool valid,
+                                               const TxSlot (&sl)[NE], int nsl, int lo, int hi) {
+  bool ok = true;
+  if (valid && p.z > 0.0f && lo <= hi) {
+    float tmin = sl[0].tau, tmax = sl[0].tau;
+#pragma unroll
+    for (int s = 1; s < NE; ++s)
+      if (s < nsl) {
+        tmin = fminf(tmin, sl[s].tau);
+        tmax = fmaxf(tmax, sl[s].tau);
+      }
+    const float rmin = p.z * A.inv_c;
+    const float rmax = fmaxf(rx_time(A.ex[lo], p, A.c, A.inv_c), rx_time(A.ex[hi], p, A.c, A.inv_c));
+    const float ilo = (tmin + rmin - A.t0) * A.fs;
+    const float ihi = (tmax + rmax - A.t0) * A.fs;
+    ok = ilo >= 2.0f && ihi <= static_cast<float>(A.n_s - 1) - 2.0f && p.z >= 1e-12f;
+  }
+  return __builtin_amdgcn_readfirstlane(wave_min(ok ? 1 : 0)) != 0;
+}
+
 // Sum over the event slots of one channel, then the receive rotation and
 // apodization: this lane's Y_ch (see the header). `ch` is wavefront-uniform,
 // and so is the event of slot s unless SEL (Select compounding: the slot's
 // own event): each slot's trace then starts at a scalar address and only the
 // sample index is per lane.
-template <int NE, bool IQ, bool SEL>
+template <int NE, bool IQ, bool SEL, bool CHK = true>
 __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
                                                 const TxSlot (&sl)[NE], int nsl, int ch, float a,
                                                 int ev_first) {
-  const float trx = rx_time(A.ex[ch], p, A.c);
+  // (nsl <= NE slots are live; !CHK: gathers_inside holds for this wavefront)
+  const float trx = CHK ? rx_time(A.ex[ch], p, A.c, A.inv_c) : rx_time_normal(A.ex[ch], p, A.c, A.inv_c);
   const float nm1 = static_cast<float>(A.n_s - 1);
   float2 acc = make_float2(0.0f, 0.0f);
+  // NE = 1 and NE = 11 are launched only with exactly that many slots
+  // (launch_fused / launch_channels), so every slot is live at compile time
+  if (NE == 1 || NE == 11) nsl = NE;
   // slots in groups of G: one uniform branch per group, and inside it all
   // the group's loads issued together (a slot past nsl reads slot 0's trace
   // and is not added; slots past NE are not formed at all)
@@ -251,7 +333,7 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
         if (SEL) {
           v[q] = sample<IQ>(data, base, A.n_s, idx);  // per-lane event: per-lane trace
         } else {
-          v[q] = sample_rsrc<IQ>(trace_rsrc<IQ>(data, base, A.n_s), nm1, idx);
+          v[q] = sample_rsrc<IQ, CHK>(trace_rsrc<IQ>(data, base, A.n_s), nm1, idx);
         }
       }
 #pragma unroll
@@ -267,7 +349,7 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
   float rr = a, ri = 0.0f;
   if (A.fdem > 0.0f) {
     float sn, cs;
-    sincosf(A.w0 * trx, &sn, &cs);
+    rot_of(A.w0 * trx, &cs, &sn);
     rr = a * cs;
     ri = a * sn;
   }
@@ -297,7 +379,9 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
   const bool valid = i0 < A.g.n0 && j < A.g.n1;
   const Pt p = valid ? grid_point(A.g, i0, j) : Pt{0.0f, 0.0f, 1.0f};
   TxSlot sl[NE];
-  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, select, sl, shared_slots);
+  // Select compounding launches one slot per pixel: only NE = 1 kernels carry it
+  const bool sel = NE == 1 && select;
+  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, sel, sl, shared_slots);
 
   int lo, hi;
   aperture_range(A, p, valid, &lo, &hi);
@@ -306,12 +390,18 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
   float2 ds = das, dq = das, coh = das;
   float inc = 0.0f;
   int na = 0;
+  const RxAperture ap = rx_aperture(A, p, valid);
+  // the channel loop twice: without bounds tests when every gather of the
+  // wavefront is inside the traces, with them otherwise
+  auto channels = [&](auto checked) {
+    constexpr bool CHK = decltype(checked)::value;
   for (int ch = lo + w; ch <= hi; ch += 4) {
-    const float a = valid ? aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha) : 0.0f;
-    if (a <= 0.0f) continue;
+    bool on;
+    const float a = rx_apod(A, ap, ch, p, &on);
+    if (!on) continue;
     ++na;
-    const float2 y = select ? channel_value<NE, IQ, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                            : channel_value<NE, IQ, false>(A, data, p, sl, nsl, ch, a, ev_first);
+    const float2 y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
+                         : channel_value<NE, IQ, false, CHK>(A, data, p, sl, nsl, ch, a, ev_first);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -333,6 +423,11 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
       inc = fmaf(m, m, inc);
     }
   }
+  };
+  if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi))
+    channels(std::false_type{});
+  else
+    channels(std::true_type{});
   const float mine[NPART] = {das.x, das.y, ds.x, ds.y, dq.x, dq.y, coh.x, coh.y, inc,
                              static_cast<float>(na)};
 #pragma unroll
@@ -403,12 +498,21 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   const int nw = (A.n_ch + 31) / 32;
   for (int q = tid; q < 64 * kMaskWordsMax; q += 256) bits[q / kMaskWordsMax][q % kMaskWordsMax] = 0;
   TxSlot sl[NE];
-  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, select, sl, shared_slots);
+  // Select compounding launches one slot per pixel: only NE = 1 kernels carry it
+  const bool sel = NE == 1 && select;
+  const int nsl = tile_slots<NE>(A, p, valid, ev_first, n_slots, sel, sl, shared_slots);
 
   int lo, hi;
   aperture_range(A, p, valid, &lo, &hi);
-  const size_t pix0 = static_cast<size_t>(i0) * A.g.n1 + (j - lane);
-  if (lo <= hi) {
+  const RxAperture ap = rx_aperture(A, p, valid);
+  const int tix = tile_index(A);
+  const size_t pix = static_cast<size_t>(i0) * A.g.n1 + j;  // this lane's pixel (when valid)
+  // the channel loop twice: without bounds tests when every gather of the
+  // wavefront is inside the traces, with them otherwise (gathers_inside is
+  // wavefront-uniform and the four wavefronts share pixels and channel range,
+  // so all take the same copy and meet the same barriers)
+  auto channels = [&](auto checked) {
+    constexpr bool CHK = decltype(checked)::value;
     for (int c0 = lo & ~31; c0 <= hi; c0 += 32) {
       // each wave: channels c0 + w + 4k of this lane's pixel; their mask bits
       // (bit cc of word c0 / 32) gathered in a register, one LDS OR per block
@@ -417,12 +521,16 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
         const int ch = c0 + cc;
         float2 y = make_float2(0.0f, 0.0f);
         if (valid && ch >= lo && ch <= hi) {
-          float a = aperture_weight(A.ex[ch], p, A.f_number, A.rx_window, A.tukey_alpha);
-          if (rect) a = a > 0.0f ? 1.0f : 0.0f;
-          if (a > 0.0f) {
+          bool on;
+          float a = rx_apod(A, ap, ch, p, &on);
+          if (rect) {  // beamformer.cpp:385: a > 0 ? 1 : 0, so a NaN weight is inactive here
+            on = on && a == a;
+            a = 1.0f;
+          }
+          if (on) {
             mybits |= 1u << cc;
-            y = select ? channel_value<NE, IQ, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                       : channel_value<NE, IQ, false>(A, data, p, sl, nsl, ch, a, ev_first);
+            y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
+                    : channel_value<NE, IQ, false, CHK>(A, data, p, sl, nsl, ch, a, ev_first);
           }
         }
         tile[lane][cc] = y;
@@ -433,9 +541,10 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
       const int ncols = min(32, A.n_ch - c0);
       for (int q = tid; q < 64 * 32; q += 256) {
         const int r = q >> 5, cc = q & 31;
-        const int jr = j - lane + r;
-        if (cc < ncols && jr < A.g.n1 && i0 < A.g.n0) {
-          float2* dst = Y + (pix0 + r) * A.n_ch + c0 + cc;
+        int ir, jr;
+        tile_pixel(A, tix, r, &ir, &jr);
+        if (cc < ncols && jr < A.g.n1 && ir < A.g.n0) {
+          float2* dst = Y + (static_cast<size_t>(ir) * A.g.n1 + jr) * A.n_ch + c0 + cc;
           float2 v = tile[r][cc];
           if (!first) {
             const float2 prev = *dst;
@@ -447,10 +556,16 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
       }
       __syncthreads();
     }
+  };
+  if (lo <= hi) {
+    if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi))
+      channels(std::false_type{});
+    else
+      channels(std::true_type{});
   }
   __syncthreads();
   if (valid && w == 0) {
-    uint32_t* mp = mask + (pix0 + lane) * nw;
+    uint32_t* mp = mask + pix * nw;
     for (int q = 0; q < nw; ++q) mp[q] = bits[lane][q];
   }
 }
@@ -542,7 +657,9 @@ __global__ void fill_zero_kernel(float* p, size_t n) {
 // Launchers
 // ---------------------------------------------------------------------------
 
-static int tile_blocks(const BfArgs& A) { return A.g.n0 * ((A.g.n1 + 63) / 64); }
+static int tile_blocks(const BfArgs& A) {
+  return ((A.g.n0 + kTileLines - 1) / kTileLines) * ((A.g.n1 + kTileDepths - 1) / kTileDepths);
+}
 
 template <int NE, bool IQ>
 static void launch_fused_ne(const BfArgs& A, const void* data, float2* frame, int ev_first,

@@ -293,6 +293,39 @@ struct BfTables {
   const float* c0 = nullptr;
 };
 
+// Where the C library's cosf returns exactly 1.0f on [0, 2 pi]: x <= lo and
+// x >= hi. The reference's Hann weight 0.5 (1 - cos(2 pi t)) (types.cpp:133)
+// is exactly 0 there, so these bounds decide its receive aperture. Found by
+// walking the floats from 2^-12 (where a correctly rounded cosine leaves 1.0f)
+// and down from (2 pi)f, once per process, with the same libm the host path
+// uses; volatile keeps the compiler from evaluating cosf itself.
+struct CosOne {
+  float lo, hi;
+};
+CosOne cos_one_bounds() {
+  static const CosOne bounds = [] {
+    auto cosf_lib = [](float x) {
+      volatile float v = x;
+      return std::cos(static_cast<float>(v));
+    };
+    CosOne b;
+    float x = std::ldexp(1.0f, -12);
+    while (cosf_lib(std::nextafter(x, 1.0f)) == 1.0f) x = std::nextafter(x, 1.0f);
+    while (x > 0.0f && cosf_lib(x) != 1.0f) x = std::nextafter(x, 0.0f);
+    b.lo = x;
+    const float two_pi = 2.0f * usxd::kPi;
+    float y = two_pi;
+    if (cosf_lib(y) != 1.0f) {
+      b.hi = INFINITY;  // no float near 2 pi gives 1.0f
+    } else {
+      while (cosf_lib(std::nextafter(y, 0.0f)) == 1.0f) y = std::nextafter(y, 0.0f);
+      b.hi = y;
+    }
+    return b;
+  }();
+  return bounds;
+}
+
 BfArgs args_of(const usx_ctx* c, const usx_bf_desc* b, const BfTables& t) {
   BfArgs A{};
   A.n_ch = b->n_channels;
@@ -313,6 +346,10 @@ BfArgs args_of(const usx_ctx* c, const usx_bf_desc* b, const BfTables& t) {
   A.tukey_alpha = b->tukey_alpha;
   A.transmit_weighting = b->transmit_weighting;
   A.n_xcd = c->n_xcd;
+  A.inv_c = 1.0f / b->speed_of_sound;
+  const CosOne one = cos_one_bounds();
+  A.cos1_lo = one.lo;
+  A.cos1_hi = one.hi;
   A.ex = t.ex;
   A.tx = t.tx;
   A.g.type = b->grid_type;

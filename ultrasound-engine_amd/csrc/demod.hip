@@ -2,14 +2,23 @@
 //
 // Reference: demodulate, core/src/demodulate.cpp:36-121.
 //
-// One workgroup per (event, channel) trace. The trace is read once from HBM
-// (coalesced), mixed with the host-tabulated rotator exp(-j 2 pi fdem t)
-// (demodulate.cpp:72-76, 98-99) into LDS, and every output sample k is then
-// the FIR sum over taps j = j_lo..j_hi in ascending order
-//   iq[k] = 2 * sum_j mixed[k*dec + gd - j] * h[j]   (demodulate.cpp:106-116)
+// Every output sample k of a trace is
+//   iq[k] = 2 * sum_j mixed[k*dec + gd - j] * h[j],  j = j_lo..j_hi ascending
+// (demodulate.cpp:106-116), mixed[n] = rotator[n] * rf[n] with the
+// host-tabulated rotator exp(-j 2 pi fdem t) (demodulate.cpp:72-76, 98-99),
 // with the reference's edge clipping and the same multiply/add sequence, so
-// the output is bitwise the reference's. Traces longer than the LDS budget
-// take the same arithmetic straight from global memory.
+// the output is bitwise the reference's.
+//
+// demod_tile_kernel (33 taps, decimation 1, 2 or 4: the configurations'
+// filters): one wavefront per 64 consecutive outputs of one trace. It mixes
+// the input window those outputs read (64 dec + 32 samples, coalesced) into
+// LDS split by decimation phase -- sample i of the window at
+// [i mod dec][i / dec] -- so that for each tap j all 64 lanes read
+// consecutive entries of one phase row (no bank conflicts) at an offset the
+// compiler knows. Interior tiles run the 33 taps without edge tests.
+// demod_lds_kernel / demod_global_kernel: any other filter length or
+// decimation (one workgroup per trace, or straight from global memory for
+// traces longer than the LDS budget).
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -27,6 +36,63 @@ __device__ __forceinline__ float2 fir_tap(float2 acc, float2 m, float h) {
 }
 
 }  // namespace
+
+template <int DEC, int NT>
+__global__ __launch_bounds__(64) void demod_tile_kernel(int n_in, int n_out,
+                                                        const float* __restrict__ taps,
+                                                        const float2* __restrict__ rot,
+                                                        const float* __restrict__ rf,
+                                                        float2* __restrict__ iq) {
+  constexpr int GD = NT / 2;
+  constexpr int W = 63 * DEC + NT;            // input samples the tile reads
+  constexpr int S = (W + DEC - 1) / DEC;      // entries per phase row
+  __shared__ float2 win[DEC * S];
+  const int lane = threadIdx.x;
+  const int k0 = blockIdx.y * 64;
+  const size_t tr = blockIdx.x;
+  const int n_lo = k0 * DEC + GD - (NT - 1);  // input index of window sample 0
+  const float* x = rf + tr * n_in;
+  for (int i = lane; i < W; i += 64) {
+    const int n = n_lo + i;
+    float2 m = make_float2(0.0f, 0.0f);
+    if (n >= 0 && n < n_in) {
+      const float v = x[n];
+      const float2 r = rot[n];
+      m = make_float2(r.x * v, r.y * v);
+    }
+    win[(i % DEC) * S + i / DEC] = m;
+  }
+  __syncthreads();
+  const int k = k0 + lane;
+  // output k, tap j reads window sample lane*DEC + (NT - 1 - j)
+  float2 acc = make_float2(0.0f, 0.0f);
+  const bool interior = n_lo >= 0 && n_lo + W <= n_in;  // every lane's j range is 0..NT-1
+  if (interior) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int i = NT - 1 - j;
+      const float2 m = win[(i % DEC) * S + i / DEC + lane];
+      const float h = taps[j];
+      acc.x += m.x * h;
+      acc.y += m.y * h;
+    }
+  } else {
+    const int c = k * DEC + GD;
+    const int j0 = max(0, c - n_in + 1);
+    const int j1 = min(NT - 1, c);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int i = NT - 1 - j;
+      const float2 m = win[(i % DEC) * S + i / DEC + lane];
+      const float h = taps[j];
+      if (j >= j0 && j <= j1) {
+        acc.x += m.x * h;
+        acc.y += m.y * h;
+      }
+    }
+  }
+  if (k < n_out) iq[tr * n_out + k] = make_float2(acc.x * 2.0f, acc.y * 2.0f);
+}
 
 __global__ __launch_bounds__(256) void demod_lds_kernel(int n_in, int n_out, int dec, int n_taps,
                                                         const float* __restrict__ taps,
@@ -81,6 +147,16 @@ __global__ __launch_bounds__(256) void demod_global_kernel(int n_traces, int n_i
 
 void launch_demod(int n_traces, int n_in, int n_out, int dec, int n_taps, const float* taps,
                   const float2* rotator, const float* rf, float2* iq, hipStream_t st) {
+  if (n_taps == 33 && (dec == 1 || dec == 2 || dec == 4)) {
+    const dim3 grid(n_traces, (n_out + 63) / 64);
+#define USX_DEMOD(D) \
+  hipLaunchKernelGGL((demod_tile_kernel<D, 33>), grid, dim3(64), 0, st, n_in, n_out, taps, rotator, rf, iq)
+    if (dec == 1) USX_DEMOD(1);
+    else if (dec == 2) USX_DEMOD(2);
+    else USX_DEMOD(4);
+#undef USX_DEMOD
+    return;
+  }
   if (n_in <= kDemodLdsMax) {
     hipLaunchKernelGGL(demod_lds_kernel, dim3(n_traces), dim3(256),
                        static_cast<size_t>(n_in) * sizeof(float2), st, n_in, n_out, dec, n_taps,

@@ -28,6 +28,9 @@ struct BfArgs {
   float tukey_alpha;
   int transmit_weighting;
   int n_xcd;                 // XCDs of the device: the tile remap gives each a contiguous range
+  float inv_c;               // 1 / speed of sound, correctly rounded (divisions by c: div_r)
+  float cos1_lo, cos1_hi;    // the C library's cosf(x) is exactly 1.0f for x <= lo and x >= hi
+                             // on [0, 2 pi]: where the Hann receive weight is exactly 0
 };
 
 // demod.hip
