@@ -12,7 +12,9 @@ Workload (one step = one frame through the hot path, RF already in HBM):
 The headline `value` is MV frames/s. DAS, DAS + coherence factor and DMAS
 frame rates on the same RF, the end-to-end (host RF in, host frame out)
 MV rate, the learned-apodization rate (trained network shipped in
-usx/data, with its gCNR on the cyst beside MV's and its BF16 MFMA roofline)
+usx/data, with its gCNR on the cyst beside MV's, its BF16 MFMA roofline and
+gCNR, speckle SNR, lateral FWHM and IQ error against MV on the 20 held-out
+phantoms, seeds 300-319)
 the uneven batched stream (64 frames of seed 1, DAS + MV each, one HIP
 graph) and colour flow / power Doppler / wall filter on a 16-frame ensemble
 of the benchmark grid are reported under "extras", with the MV, DAS and
@@ -458,6 +460,8 @@ def main():
     ap.add_argument("--pmc-combiner", default="MV", choices=["MV", "DAS", "DMAS", "Learned"],
                     help="combiner of the --pmc-workload frames")
     ap.add_argument("--skip-stream", action="store_true", help="omit the uneven-stream leg")
+    ap.add_argument("--skip-eval", action="store_true",
+                    help="omit the 20-phantom learned-vs-MV image-quality evaluation")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the uneven-stream leg and print its result (for profiling)")
     ap.add_argument("--stream-frames", type=int, default=64)
@@ -530,6 +534,11 @@ def main():
                                      "flops_per_launch": infer_flops, "kernel_ms": infer_ms}
     log(f"  learned: {l_ms:.4f} ms/frame ({mv_ms / l_ms:.1f}x MV), kernels {lk}, "
         f"gCNR {extras['gcnr_learned']:.3f} vs MV {extras['gcnr_mv']:.3f}")
+    if not a.skip_eval:
+        log("learned vs MV on the 20 held-out phantoms (seeds 300-319) ...")
+        ev = L.evaluate(apod, list(range(300, 320)), verbose=False)["summary"]
+        extras["learned_eval_20_seeds"] = ev
+        log(f"  {ev}")
     extras["mv_kernels_ms"] = {n: v[1] / v[0] for n, v in kern.items()}
 
     flops = mv_reference_flops(probe, grid)

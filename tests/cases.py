@@ -73,3 +73,24 @@ def oracle_inputs(O, case, data=None):
     s = O.Setup.of(case.probe, case.medium, case.acq)
     txs = [O.Tx.of(t) for t in case.txs]
     return s, txs
+
+
+def wires(usx, points, amplitude=1.0):
+    """Point scatterers at (x, z) positions in the plane y = 0."""
+    return usx.Phantom.point_targets([usx.Vec3(float(x), 0.0, float(z)) for x, z in points],
+                                     amplitude)
+
+
+def speckle_with_cysts(usx, probe, medium, cysts, z_min, z_max, per_cell=12.0, seed=3):
+    """Gaussian speckle over 1.15 x the aperture width and [z_min, z_max] at
+    `per_cell` scatterers per resolution cell (f/2 cell, 0.4 mm thick slab),
+    with each (x, z, radius, amplitude scale) circle rescaled."""
+    half = 0.575 * probe.aperture_width()
+    n = usx.Phantom.recommended_scatterer_count(probe, medium, 2 * half, z_max - z_min, 4e-4,
+                                                f_number=2.0, per_cell=per_cell)
+    ph = usx.Phantom.speckle_box(-half, half, z_min, z_max, 2e-4, int(min(max(n, 2000), 400000)),
+                                 seed)
+    for x, z, r, scale in cysts:
+        ph.scale_amplitude_in_circle(usx.Vec3(float(x), 0.0, float(z)), float(r), float(scale))
+    return ph
+

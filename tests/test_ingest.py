@@ -1,4 +1,4 @@
-"""Raw-data sources and the .npz wire format (usx.ingest; reference
+"""Raw-data sources and the .npz recording format (usx.ingest; reference
 python/usx/ingest.py and its tests in tests/test_usx.py:159-203).
 
 The archive must hold exactly the reference's keys with the reference's
@@ -10,6 +10,8 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+
+from cases import wires
 
 REFERENCE_KEYS = {
     "rf": np.float32, "probe_n_elements": np.integer, "probe_pitch": np.floating,
@@ -37,7 +39,7 @@ def setup(usx_mod):
 def test_round_trip_keeps_samples_and_geometry(usx_mod, setup, tmp_path):
     usx = usx_mod
     probe, medium, acq, cfg, sim = setup
-    ph = usx.phantoms.wire_targets(depths=(0.025,), lateral=(0.0,))
+    ph = wires(usx, [(0.0, 0.025)])
     txs = sim.make_plane_waves([0.0])
     src = usx.ingest.SimulatedSource(probe, medium, acq, ph, txs, config=cfg, n_frames=2)
     frames = list(src.frames())
@@ -100,7 +102,7 @@ def test_multi_frame_archive_and_loop(usx_mod, setup, tmp_path):
 def test_moving_phantom_advances_between_frames(usx_mod, setup):
     usx = usx_mod
     probe, medium, acq, cfg, sim = setup
-    ph = usx.phantoms.wire_targets(depths=(0.02,), lateral=(0.0,))
+    ph = wires(usx, [(0.0, 0.02)])
     ph.velocities = np.array([[0.0, 0.0, 0.5]], np.float32)
     txs = sim.make_plane_waves([0.0])
     frames = list(usx.ingest.SimulatedSource(probe, medium, acq, ph, txs, config=cfg,
@@ -123,16 +125,50 @@ def test_inconsistent_transmit_vectors_are_rejected(usx_mod, setup, tmp_path):
 
 
 @pytest.mark.gpu
-def test_replayed_data_beamforms_to_the_same_image(usx_mod, setup, tmp_path):
+def test_replayed_recording_beamforms_like_the_reference(usx_mod, setup, tmp_path, port, O):
+    """A recording replayed through FileSource gives the GPU pipeline the same
+    frame as the original RF (bit for bit), and that frame matches the oracle's
+    restatement of the reference on the original RF (1e-5 relative L2)."""
+    from conftest import rel_l2
     usx = usx_mod
     probe, medium, acq, cfg, sim = setup
-    ph = usx.phantoms.wire_targets(depths=(0.025,), lateral=(0.0,))
-    txs = sim.make_plane_waves([0.0])
+    ph = wires(usx, [(0.0, 0.025), (0.004, 0.02)], amplitude=5.0)
+    ph.extend(usx.Phantom.speckle_box(-0.008, 0.008, 0.01, 0.035, 2e-4, 800, 11, 1.0))
+    txs = sim.make_plane_waves(usx.angles_deg(-6, 6, 3))
     raw = sim.simulate(txs, ph)
     path = usx.ingest.save_npz(tmp_path / "acq.npz", raw)
     replayed = next(usx.ingest.FileSource(path).frames())
-    grid = usx.ScanGrid.cartesian(-0.004, 0.004, 41, 0.022, 0.028, 61)
-    bf = usx.Beamformer()
-    a = np.asarray(usx.envelope(bf.beamform(usx.demodulate(raw, usx.DemodConfig()), grid)))
-    b = np.asarray(usx.envelope(bf.beamform(usx.demodulate(replayed, usx.DemodConfig()), grid)))
-    assert np.array_equal(a, b)
+    grid = usx.ScanGrid.cartesian(-0.006, 0.006, 40, 0.012, 0.034, 90)
+    pc = usx.PipelineConfig()
+    pc.demod.decimation = 2
+    for combiner in ("DAS", "MV"):
+        b = pc.beamformer
+        b.combiner = getattr(usx.Combiner, combiner)
+        b.mv_subarray = 16
+        pc.beamformer = b
+        a = np.asarray(usx.ImagingPipeline(grid, pc).process_frame(raw).data)
+        r = np.asarray(usx.ImagingPipeline(grid, pc).process_frame(replayed).data)
+        assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), combiner
+        ref, _ = port.process_frame(O.Setup.of(probe, medium, acq), [O.Tx.of(t) for t in txs],
+                                    np.asarray(raw.rf), O.GridSpec(0, -0.006, 0.006, 40, 0.012,
+                                                                   0.034, 90),
+                                    O.Demod(decimation=2), O.Bf.of(b))
+        assert rel_l2(r, ref) < (1e-4 if combiner == "MV" else 1e-5), combiner
+
+
+@pytest.mark.gpu
+def test_simulated_source_on_the_device(usx_mod, setup):
+    """SimulatedSource(device=True) renders with the GPU simulator: the same
+    frames as the host simulator's to float rounding."""
+    from conftest import rel_l2
+    usx = usx_mod
+    probe, medium, acq, cfg, sim = setup
+    ph = wires(usx, [(0.0, 0.02), (0.003, 0.03)])
+    ph.velocities = np.array([[0.0, 0.0, 0.5], [0.0, 0.0, -0.2]], np.float32)
+    txs = sim.make_plane_waves([0.0, 0.1])
+    host = list(usx.ingest.SimulatedSource(probe, medium, acq, ph, txs, config=cfg,
+                                           n_frames=2).frames())
+    dev = list(usx.ingest.SimulatedSource(probe, medium, acq, ph, txs, config=cfg, n_frames=2,
+                                          device=True).frames())
+    for h, d in zip(host, dev):
+        assert rel_l2(np.asarray(d.rf), np.asarray(h.rf)) < 1e-5

@@ -21,7 +21,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cases import bf_config
+from cases import bf_config, speckle_with_cysts, wires
 
 pytestmark = pytest.mark.gpu
 
@@ -220,8 +220,7 @@ def test_frame_is_complex_and_localizes(usx_mod, setup64):
 def test_bmode_is_normalized(usx_mod, setup64):
     usx = usx_mod
     probe, med, acq, sim = setup64
-    cd = sim.simulate(sim.make_plane_waves([0.0]), usx.phantoms.wire_targets(depths=(0.02, 0.03),
-                                                                               lateral=(0.0,)))
+    cd = sim.simulate(sim.make_plane_waves([0.0]), wires(usx, [(0.0, 0.02), (0.0, 0.03)]))
     grid = usx.grid_for(probe, 0.01, 0.04, 64, 160)
     pc = usx.PipelineConfig()
     pc.dynamic_range_db = 50.0
@@ -249,8 +248,7 @@ def test_resolution_improves_with_more_angles(usx_mod, setup64):
 def test_speckle_snr_matches_rayleigh(usx_mod, setup64):
     usx = usx_mod
     probe, med, acq, sim = setup64
-    ph = usx.phantoms.cyst_phantom(probe, med, cysts=(), z_min=0.010, z_max=0.035,
-                                   density_per_cell=15.0)
+    ph = speckle_with_cysts(usx, probe, med, (), 0.010, 0.035, per_cell=15.0)
     iq = usx.demodulate(sim.simulate_gpu(usx.presets.plane_wave_sequence(sim, 9.0, 5), ph),
                         usx.DemodConfig())
     grid = usx.grid_for(probe, 0.015, 0.030, 96, 240)
@@ -262,8 +260,7 @@ def test_speckle_snr_matches_rayleigh(usx_mod, setup64):
 def test_cyst_is_darker_than_background(usx_mod, setup64):
     usx = usx_mod
     probe, med, acq, sim = setup64
-    ph = usx.phantoms.cyst_phantom(probe, med, cysts=((0.0, 0.025, 0.003, 0.0),),
-                                   z_min=0.010, z_max=0.038)
+    ph = speckle_with_cysts(usx, probe, med, ((0.0, 0.025, 0.003, 0.0),), 0.010, 0.038)
     iq = usx.demodulate(sim.simulate_gpu(usx.presets.plane_wave_sequence(sim, 9.0, 7), ph),
                         usx.DemodConfig())
     grid = usx.grid_for(probe, 0.015, 0.035, 128, 240)
@@ -275,8 +272,7 @@ def test_cyst_is_darker_than_background(usx_mod, setup64):
 def test_pipeline_reuses_a_smoothed_reference(usx_mod, setup64):
     usx = usx_mod
     probe, med, acq, sim = setup64
-    cd = sim.simulate(sim.make_plane_waves([0.0]), usx.phantoms.wire_targets(depths=(0.025,),
-                                                                              lateral=(0.0,)))
+    cd = sim.simulate(sim.make_plane_waves([0.0]), wires(usx, [(0.0, 0.025)]))
     pipe = usx.ImagingPipeline(usx.grid_for(probe, 0.015, 0.035, 48, 120))
     a = np.asarray(pipe.process_display(cd))
     b = np.asarray(pipe.process_display(cd))

@@ -25,7 +25,6 @@
 #include "usx/doppler.hpp"
 #include "usx/geometry.hpp"
 #include "usx/pipeline.hpp"
-#include "usx/postprocess.hpp"
 #include "usx/simulator.hpp"
 #include "usx/types.hpp"
 #include "usx_gpu.h"
@@ -1074,36 +1073,6 @@ PYBIND11_MODULE(_usx, m) {
            py::arg("grid"), py::arg("events_per_frame"), py::call_guard<py::gil_scoped_release>())
       .def_property("config", py::overload_cast<>(&Beamformer::config, py::const_),
                     [](Beamformer& b, const BeamformerConfig& c) { b.config() = c; });
-
-  m.def(
-      "envelope",
-      [](const Frame& f) {
-        std::vector<real> e = envelope(f);
-        return copy_vector(e, {f.grid.n_axis0, f.grid.n_axis1});
-      },
-      py::arg("frame"));
-  m.def(
-      "log_compress",
-      [](py::array_t<real, py::array::c_style | py::array::forcecast> env, real dr, real gain,
-         real reference) {
-        std::vector<real> v(env.data(), env.data() + env.size());
-        std::vector<real> out = log_compress(v, dr, gain, reference);
-        py::array_t<real> a(std::vector<py::ssize_t>(env.shape(), env.shape() + env.ndim()));
-        std::memcpy(a.mutable_data(), out.data(), out.size() * sizeof(real));
-        return a;
-      },
-      py::arg("envelope"), py::arg("dynamic_range_db") = 60.0f, py::arg("gain_db") = 0.0f,
-      py::arg("reference") = -1.0f);
-  m.def(
-      "apply_tgc",
-      [](py::array_t<real, py::array::c_style | py::array::forcecast> env, const ScanGrid& grid,
-         real atten, real f0) {
-        std::vector<real> v(env.data(), env.data() + env.size());
-        std::vector<real> out = apply_tgc(v, grid, atten, f0);
-        return copy_vector(out, {grid.n_axis0, grid.n_axis1});
-      },
-      py::arg("envelope"), py::arg("grid"), py::arg("attenuation_db_cm_mhz"),
-      py::arg("center_frequency_hz"));
 
   py::class_<PipelineConfig>(m, "PipelineConfig")
       .def(py::init<>())

@@ -33,7 +33,7 @@ BUILD = os.path.join(PKG, "_build")
 HIP_SOURCES = ["context.hip", "demod.hip", "beamform.hip", "mv.hip", "apod.hip", "sim.hip", "display.hip",
                "doppler.hip"]
 CORE_SOURCES = ["types.cpp", "geometry.cpp", "gpu.cpp", "beamformer.cpp", "pipeline.cpp", "doppler.cpp",
-                "postprocess.cpp", "simulator.cpp"]
+                "simulator.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -11,7 +11,6 @@
 
 #include "usx/beamformer.hpp"
 #include "usx/demodulate.hpp"
-#include "usx/postprocess.hpp"
 #include "usx/types.hpp"
 
 namespace usx {

@@ -52,15 +52,22 @@ __global__ __launch_bounds__(64) void demod_tile_kernel(int n_in, int n_out,
   const size_t tr = blockIdx.x;
   const int n_lo = k0 * DEC + GD - (NT - 1);  // input index of window sample 0
   const float* x = rf + tr * n_in;
-  for (int i = lane; i < W; i += 64) {
+  // every load of the window issued before any is used: one memory round trip
+  constexpr int NL = (W + 63) / 64;
+  float v[NL];
+  float2 r[NL];
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int i = lane + 64 * q;
     const int n = n_lo + i;
-    float2 m = make_float2(0.0f, 0.0f);
-    if (n >= 0 && n < n_in) {
-      const float v = x[n];
-      const float2 r = rot[n];
-      m = make_float2(r.x * v, r.y * v);
-    }
-    win[(i % DEC) * S + i / DEC] = m;
+    const bool in = i < W && n >= 0 && n < n_in;
+    v[q] = in ? x[n] : 0.0f;
+    r[q] = in ? rot[n] : make_float2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int i = lane + 64 * q;
+    if (i < W) win[(i % DEC) * S + i / DEC] = make_float2(r[q].x * v[q], r[q].y * v[q]);
   }
   __syncthreads();
   const int k = k0 + lane;

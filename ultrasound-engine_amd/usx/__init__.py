@@ -54,7 +54,6 @@ from ._usx import (  # noqa: F401
     WallFilter,
     Window,
     __version__,
-    apply_tgc,
     apply_wall_filter,
     axial_strain,
     build_info,
@@ -64,9 +63,7 @@ from ._usx import (  # noqa: F401
     design_lowpass,
     dynamic_aperture_weight,
     element_directivity,
-    envelope,
     estimate_axial_displacement,
-    log_compress,
     polynomial_reject_matrix,
     power_doppler,
     transmit_arrival_time,
@@ -74,7 +71,7 @@ from ._usx import (  # noqa: F401
     window_value,
 )
 
-from . import ingest, metrics, phantoms, presets  # noqa: E402,F401
+from . import ingest, metrics, presets  # noqa: E402,F401
 
 __all__ = [n for n in dir() if not n.startswith("_")] + [
     "angles_deg", "to_bmode", "grid_for", "extent_mm",
@@ -92,13 +89,16 @@ def angles_deg(start: float, stop: float, count: int) -> list[float]:
 def to_bmode(frame: Frame, dynamic_range_db: float = 60.0, gain_db: float = 0.0,
              medium: Medium | None = None, tgc: bool = True) -> np.ndarray:
     """Complex frame to a log-compressed image in [0, 1], shape (axis0, axis1)
-    (reference: python/usx/__init__.py:93-105)."""
-    env = envelope(frame)
-    if tgc:
-        med = medium or Medium()
-        f0 = frame.demodulation_frequency or 5.0e6
-        env = apply_tgc(env, frame.grid, med.attenuation, f0)
-    return log_compress(env, dynamic_range_db, gain_db, -1.0)
+    (reference: python/usx/__init__.py:93-105): envelope, TGC at the frame's
+    demodulation frequency (5 MHz for RF frames) and log compression against
+    the frame's own peak, on the GPU display path (ImagingPipeline.to_display
+    of a fresh pipeline, whose first reference level is that peak)."""
+    pc = PipelineConfig()
+    pc.apply_tgc = tgc
+    pc.dynamic_range_db = dynamic_range_db
+    pc.gain_db = gain_db
+    img = ImagingPipeline(frame.grid, pc).to_display(frame, medium or Medium())
+    return np.asarray(img, dtype=np.float32).reshape(frame.grid.n_axis0, frame.grid.n_axis1)
 
 
 def grid_for(probe: Probe, z_min: float = 0.005, z_max: float = 0.055,

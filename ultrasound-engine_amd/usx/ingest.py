@@ -1,31 +1,63 @@
-"""Raw channel-data sources (reference: python/usx/ingest.py:1-233).
+"""Raw channel data in: sources and the .npz recording format.
 
-Every source answers the same two questions -- what acquired this data
-(``metadata``: probe, medium, acquisition, transmits) and what are its frames
-(``frames()``: an iterator of ``ChannelData``) -- so the pipeline does not care
-whether RF was simulated, replayed from a file or handed over by a scanner
-driver. The archive format of ``save_npz`` / ``FileSource`` is the
-reference's, key for key, so recordings made with either package load in the
-other.
+A source is anything with ``metadata`` (probe, medium, acquisition,
+transmits) and ``frames()`` (an iterator of ChannelData, one frame's events
+each), the interface of the reference's python/usx/ingest.py:36-62, so a
+pipeline is written once whether its RF was simulated, replayed from a
+recording or handed over by acquisition hardware as arrays.
 
-Loading never unpickles: archives are read with ``numpy.load(...,
-allow_pickle=False)``.
+The recording format is the reference's (python/usx/ingest.py:201-233): one
+``np.savez_compressed`` archive with the RF samples and every parameter needed
+to beamform them, under the key names and dtypes of ``ARCHIVE`` below. That
+table is the only part kept alike, so that recordings move between the two
+packages; saving and loading are both driven by it.
+
+Sources here add two things: ``SimulatedSource(..., device=True)`` renders the
+frames with the GPU simulator, and ``FileSource`` keeps the archive's samples
+as one (frames, events, channels, samples) view and hands out frames without
+copying them again.
 """
 
 from __future__ import annotations
 
-import abc
 from dataclasses import dataclass, field
 from pathlib import Path
+from typing import Iterator
 
 import numpy as np
 
-from ._usx import (Acquisition, ChannelData, Medium, Phantom, Probe, Simulator,
-                   SimulatorConfig, Transmit, TransmitType, Vec3)
+from ._usx import (Acquisition, ChannelData, Medium, Phantom, Probe, Simulator, SimulatorConfig,
+                   Transmit, TransmitType, Vec3)
 
-# Archive code of each transmit type (the reference's order).
-_TYPE_CODES = {TransmitType.FOCUSED: 0, TransmitType.PLANE_WAVE: 1, TransmitType.DIVERGING: 2}
-_CODE_TYPES = {v: k for k, v in _TYPE_CODES.items()}
+# Transmit type codes of the archive's tx_type column.
+TX_CODES = {TransmitType.FOCUSED: 0, TransmitType.PLANE_WAVE: 1, TransmitType.DIVERGING: 2}
+TX_TYPES = {code: kind for kind, code in TX_CODES.items()}
+
+# key -> (owner, attribute, dtype). Scalars are stored as 0-d arrays of the
+# dtype numpy gives the Python value (int64 / float64), per-transmit columns
+# with the dtype named here, as the reference writes them.
+ARCHIVE = {
+    "probe_n_elements": ("probe", "n_elements", int),
+    "probe_pitch": ("probe", "pitch", float),
+    "probe_element_width": ("probe", "element_width", float),
+    "probe_elevation_height": ("probe", "elevation_height", float),
+    "probe_center_frequency": ("probe", "center_frequency", float),
+    "probe_bandwidth": ("probe", "bandwidth", float),
+    "medium_speed_of_sound": ("medium", "speed_of_sound", float),
+    "medium_attenuation": ("medium", "attenuation", float),
+    "acq_sampling_frequency": ("acq", "sampling_frequency", float),
+    "acq_n_samples": ("acq", "n_samples", int),
+    "acq_t0": ("acq", "t0", float),
+    "acq_prf": ("acq", "prf", float),
+}
+TX_COLUMNS = {
+    "tx_type": (np.int32, lambda t: TX_CODES[t.type]),
+    "tx_angle": (np.float32, lambda t: t.angle),
+    "tx_focus": (np.float32, lambda t: (t.focus.x, t.focus.y, t.focus.z)),
+    "tx_origin": (np.float32, lambda t: (t.origin.x, t.origin.y, t.origin.z)),
+    "tx_delays": (np.float32, lambda t: list(t.delays)),
+    "tx_apodization": (np.float32, lambda t: list(t.apodization)),
+}
 
 
 @dataclass
@@ -36,17 +68,19 @@ class SourceMetadata:
     transmits: list = field(default_factory=list)
 
 
-class RawSource(abc.ABC):
-    """A source of pre-beamforming channel data."""
+class RawSource:
+    """Base of every source: subclasses provide ``metadata`` and ``_blocks()``
+    (float32 arrays of shape (events, channels, samples), one per frame)."""
 
-    @property
-    @abc.abstractmethod
-    def metadata(self) -> SourceMetadata:
-        ...
+    metadata: SourceMetadata
 
-    @abc.abstractmethod
-    def frames(self):
-        """Yield one ``ChannelData`` per image frame."""
+    def _blocks(self) -> Iterator[np.ndarray]:
+        raise NotImplementedError
+
+    def frames(self) -> Iterator[ChannelData]:
+        m = self.metadata
+        for block in self._blocks():
+            yield ChannelData.from_rf(block, m.probe, m.medium, m.acquisition, m.transmits)
 
     def __iter__(self):
         return self.frames()
@@ -66,169 +100,116 @@ class RawSource(abc.ABC):
 
 
 class SimulatedSource(RawSource):
-    """Frames simulated from a phantom on demand; ``n_frames=None`` streams
-    forever. Moving scatterers advance by one frame period (transmits / PRF)
-    per frame, so a stream shows continuous motion."""
+    """Frames rendered from a phantom by the point-scatterer simulator,
+    ``n_frames`` of them (None: without end). Scatterers with a velocity
+    keep moving between frames: frame i starts i x (events / prf) after
+    frame 0, and the simulator moves them within each frame's events itself.
+    ``device``: render with the GPU simulator (Simulator.simulate_gpu) instead
+    of the host one, whose RF is the reference's bit for bit."""
 
     def __init__(self, probe: Probe, medium: Medium, acquisition: Acquisition, phantom: Phantom,
                  transmits: list, config: SimulatorConfig | None = None,
-                 n_frames: int | None = 1, gpu: bool = False):
-        self._sim = Simulator(probe, medium, acquisition, config or SimulatorConfig())
-        self._meta = SourceMetadata(probe, medium, acquisition, list(transmits))
+                 n_frames: int | None = 1, device: bool = False):
+        self.metadata = SourceMetadata(probe, medium, acquisition, list(transmits))
+        self.simulator = Simulator(probe, medium, acquisition, config or SimulatorConfig())
         self._phantom = phantom
         self._n_frames = n_frames
-        self._gpu = gpu
+        self._render = self.simulator.simulate_gpu if device else self.simulator.simulate
 
-    @property
-    def metadata(self) -> SourceMetadata:
-        return self._meta
-
-    @property
-    def simulator(self) -> Simulator:
-        return self._sim
-
-    def _simulate(self, phantom: Phantom) -> ChannelData:
-        if self._gpu:
-            return self._sim.simulate_gpu(self._meta.transmits, phantom)
-        return self._sim.simulate(self._meta.transmits, phantom)
-
-    def frames(self):
-        period = len(self._meta.transmits) / self._meta.acquisition.prf
+    def _phantom_at(self, i: int) -> Phantom:
+        """The phantom as it stands when frame i starts."""
         vel = np.asarray(self._phantom.velocities, dtype=np.float64)
-        moving = bool(vel.size and np.any(vel != 0.0))
-        start = np.asarray(self._phantom.positions, dtype=np.float64) if moving else None
-        f = 0
-        while self._n_frames is None or f < self._n_frames:
-            if moving and f > 0:
-                p = self._phantom.copy()
-                p.positions = (start + vel * (f * period)).astype(np.float32)
-                yield self._simulate(p)
-            else:
-                yield self._simulate(self._phantom)
-            f += 1
+        if i == 0 or not vel.any():
+            return self._phantom
+        start = i * len(self.metadata.transmits) / self.metadata.acquisition.prf
+        moved = self._phantom.copy()
+        moved.positions = (np.asarray(self._phantom.positions, dtype=np.float64)
+                           + vel * start).astype(np.float32)
+        return moved
+
+    def frames(self) -> Iterator[ChannelData]:
+        i = 0
+        while self._n_frames is None or i < self._n_frames:
+            yield self._render(self.metadata.transmits, self._phantom_at(i))
+            i += 1
+
+
+class ArraySource(RawSource):
+    """Frames handed over as arrays of shape (events, channels, samples), e.g.
+    by acquisition hardware, with the geometry to beamform them."""
+
+    def __init__(self, arrays, probe: Probe, medium: Medium, acquisition: Acquisition,
+                 transmits: list):
+        self.metadata = SourceMetadata(probe, medium, acquisition, list(transmits))
+        self._arrays = arrays
+
+    def _blocks(self):
+        for a in self._arrays:
+            yield np.asarray(a, dtype=np.float32)
 
 
 class FileSource(RawSource):
-    """Frames replayed from an archive written by :func:`save_npz`; an RF
-    array of more events than transmits holds several frames back to back."""
+    """Frames replayed from an archive written by ``save_npz``: the samples are
+    (events, channels, samples) with the events of successive frames one after
+    the other; a trailing partial frame is dropped. ``loop``: start over at
+    the end, without end."""
 
     def __init__(self, path, loop: bool = False):
         self.path = Path(path)
         self._loop = loop
         with np.load(self.path, allow_pickle=False) as z:
-            self._rf = np.asarray(z["rf"], dtype=np.float32)
-            probe = Probe()
-            probe.n_elements = int(z["probe_n_elements"])
-            probe.pitch = float(z["probe_pitch"])
-            probe.element_width = float(z["probe_element_width"])
-            probe.elevation_height = float(z["probe_elevation_height"])
-            probe.center_frequency = float(z["probe_center_frequency"])
-            probe.bandwidth = float(z["probe_bandwidth"])
-            medium = Medium()
-            medium.speed_of_sound = float(z["medium_speed_of_sound"])
-            medium.attenuation = float(z["medium_attenuation"])
-            acq = Acquisition()
-            acq.sampling_frequency = float(z["acq_sampling_frequency"])
-            acq.n_samples = int(z["acq_n_samples"])
-            acq.t0 = float(z["acq_t0"])
-            acq.prf = float(z["acq_prf"])
-            transmits = transmits_from_arrays(z["tx_type"], z["tx_angle"], z["tx_focus"],
-                                              z["tx_origin"], z["tx_delays"], z["tx_apodization"])
-        self._meta = SourceMetadata(probe, medium, acq, transmits)
-
-    @property
-    def metadata(self) -> SourceMetadata:
-        return self._meta
+            self.metadata = _metadata_from(z)
+            rf = np.asarray(z["rf"], dtype=np.float32)
+        n_ev = max(1, self.events_per_frame)
+        if rf.ndim == 3:
+            n = max(1, rf.shape[0] // n_ev)
+            self._stack = rf[: n * n_ev].reshape(n, -1, *rf.shape[1:])
+        else:
+            self._stack = rf[None]
 
     @property
     def n_frames(self) -> int:
-        n_ev = max(1, len(self._meta.transmits))
-        return max(1, self._rf.shape[0] // n_ev) if self._rf.ndim == 3 else 1
+        return self._stack.shape[0]
 
-    def frames(self):
-        n_ev = len(self._meta.transmits)
-        m = self._meta
+    def _blocks(self):
         while True:
-            for f in range(self.n_frames):
-                block = self._rf[f * n_ev:(f + 1) * n_ev] if self._rf.ndim == 3 else self._rf
-                yield ChannelData.from_rf(block, m.probe, m.medium, m.acquisition, m.transmits)
+            yield from self._stack
             if not self._loop:
                 return
 
 
-class ArraySource(RawSource):
-    """In-memory (events, channels, samples) RF blocks -- the adapter a scanner
-    driver feeds."""
-
-    def __init__(self, arrays, probe: Probe, medium: Medium, acquisition: Acquisition,
-                 transmits: list):
-        self._arrays = arrays
-        self._meta = SourceMetadata(probe, medium, acquisition, list(transmits))
-
-    @property
-    def metadata(self) -> SourceMetadata:
-        return self._meta
-
-    def frames(self):
-        m = self._meta
-        for a in self._arrays:
-            yield ChannelData.from_rf(np.asarray(a, dtype=np.float32), m.probe, m.medium,
-                                      m.acquisition, m.transmits)
-
-
-def transmits_from_arrays(types, angles, focus, origin, delays, apod) -> list:
-    """Transmit objects from the archive's per-transmit arrays."""
-    out = []
-    for i in range(len(types)):
+def _metadata_from(z) -> SourceMetadata:
+    objs = {"probe": Probe(), "medium": Medium(), "acq": Acquisition()}
+    for key, (owner, attr, kind) in ARCHIVE.items():
+        setattr(objs[owner], attr, kind(z[key]))
+    n = len(z["tx_type"])
+    transmits = []
+    for i in range(n):
         t = Transmit()
-        t.type = _CODE_TYPES[int(types[i])]
-        t.angle = float(angles[i])
-        t.focus = Vec3(*(float(v) for v in focus[i]))
-        t.origin = Vec3(*(float(v) for v in origin[i]))
-        t.delays = [float(v) for v in np.ravel(delays[i])]
-        t.apodization = [float(v) for v in np.ravel(apod[i])]
-        out.append(t)
-    return out
-
-
-def _per_transmit(rows, what: str) -> np.ndarray:
-    """Per-transmit vectors as one (n_transmits, width) float32 array; all
-    empty gives width 0 (an empty vector means "none given", which zeros
-    would not preserve, so empty and filled vectors cannot be mixed)."""
-    rows = [np.asarray(r, np.float32).ravel() for r in rows]
-    sizes = {r.size for r in rows}
-    if len(sizes) > 1:
-        raise ValueError(f"save_npz: transmits have {what} vectors of different lengths {sorted(sizes)}")
-    width = sizes.pop() if sizes else 0
-    return np.stack(rows) if rows else np.zeros((0, width), np.float32)
+        t.type = TX_TYPES[int(z["tx_type"][i])]
+        t.angle = float(z["tx_angle"][i])
+        t.focus = Vec3(*map(float, z["tx_focus"][i]))
+        t.origin = Vec3(*map(float, z["tx_origin"][i]))
+        t.delays = [float(v) for v in z["tx_delays"][i]]
+        t.apodization = [float(v) for v in z["tx_apodization"][i]]
+        transmits.append(t)
+    return SourceMetadata(objs["probe"], objs["medium"], objs["acq"], transmits)
 
 
 def save_npz(path, data: ChannelData) -> Path:
-    """Write RF channel data with everything needed to interpret it (probe,
-    medium, acquisition, transmits) as a compressed .npz archive."""
+    """Write one ChannelData (RF) with everything needed to beamform it, in the
+    reference's archive layout. Transmits must agree on the lengths of their
+    delay and apodization lists (the columns are 2-D arrays)."""
     path = Path(path)
+    objs = {"probe": data.probe, "medium": data.medium, "acq": data.acq}
+    arrays = {"rf": np.asarray(data.rf, dtype=np.float32)}
+    for key, (owner, attr, _) in ARCHIVE.items():
+        arrays[key] = getattr(objs[owner], attr)
     txs = list(data.transmits)
-    p, med, acq = data.probe, data.medium, data.acq
-    np.savez_compressed(
-        path,
-        rf=np.asarray(data.rf, dtype=np.float32),
-        probe_n_elements=p.n_elements,
-        probe_pitch=p.pitch,
-        probe_element_width=p.element_width,
-        probe_elevation_height=p.elevation_height,
-        probe_center_frequency=p.center_frequency,
-        probe_bandwidth=p.bandwidth,
-        medium_speed_of_sound=med.speed_of_sound,
-        medium_attenuation=med.attenuation,
-        acq_sampling_frequency=acq.sampling_frequency,
-        acq_n_samples=acq.n_samples,
-        acq_t0=acq.t0,
-        acq_prf=acq.prf,
-        tx_type=np.array([_TYPE_CODES[t.type] for t in txs], dtype=np.int32),
-        tx_angle=np.array([t.angle for t in txs], dtype=np.float32),
-        tx_focus=np.array([[t.focus.x, t.focus.y, t.focus.z] for t in txs], dtype=np.float32),
-        tx_origin=np.array([[t.origin.x, t.origin.y, t.origin.z] for t in txs], dtype=np.float32),
-        tx_delays=_per_transmit([t.delays for t in txs], "delay"),
-        tx_apodization=_per_transmit([t.apodization for t in txs], "apodization"),
-    )
+    for key, (dtype, get) in TX_COLUMNS.items():
+        values = [get(t) for t in txs]
+        if key in ("tx_delays", "tx_apodization") and len({len(v) for v in values}) > 1:
+            raise ValueError(f"save_npz: transmits disagree on the length of {key[3:]}")
+        arrays[key] = np.array(values, dtype=dtype)
+    np.savez_compressed(path, **arrays)
     return path

@@ -259,6 +259,13 @@ def evaluate(params, seeds, subarray: int = 64, verbose: bool = True) -> dict:
     for k in ("mv", "learned", "das_rect"):
         keys = sorted({m for r in rec for m in r[k]})
         summary[k] = {m: float(np.mean([r[k][m] for r in rec if m in r[k]])) for m in keys}
+    # per-seed agreement with minimum variance, the judge
+    d = [abs(r["learned"]["gcnr"] - r["mv"]["gcnr"]) for r in rec
+         if "gcnr" in r["learned"] and "gcnr" in r["mv"]]
+    if d:
+        summary["gcnr_delta"] = {"mean_abs": float(np.mean(d)), "max_abs": float(np.max(d)),
+                                 "of_means": abs(summary["learned"]["gcnr"] - summary["mv"]["gcnr"]),
+                                 "seeds": len(d)}
     return {"summary": summary, "records": rec}
 
 
