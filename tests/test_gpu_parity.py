@@ -346,6 +346,9 @@ def test_mv_precision_variants_against_fp64(usx_mod, port, O, config0):
         errs[(prec, refine)] = rel_l2(got, ref64)
         assert errs[(prec, refine)] < bound, (prec, refine, errs)
     assert errs[("FP16", True)] < errs[("BF16", True)]
+    # the FP64-residual step brings FP32 closer to the FP64 oracle
+    print(f"[precision] errors vs FP64: {errs}", flush=True)
+    assert errs[("FP32", True)] < errs[("FP32", False)], errs
 
 
 def test_mv_192_channels_subarray_96(usx_mod, port, O):
@@ -378,6 +381,30 @@ def test_mv_192_channels_subarray_96(usx_mod, port, O):
     bfr = bf_config(usx_mod, "MV", subarray=96, f_number=1.0, refine=True)
     got_r = np.asarray(usx_mod.Beamformer(bfr).beamform(iq, case.grid).data)
     assert rel_l2(got_r, ref) < TOL_MV, rel_l2(got_r, ref)
+
+
+def test_mv_256_channels_subarray_128(usx_mod, port, O):
+    """256-element probe, subarray 128 at f/0.5: below about 38 mm every
+    element is active and L reaches 128, so the 112- and 128-row buckets run
+    (the reference has no upper bound on L, beamformer.cpp:147-148), with and
+    without the refinement step (full matrix in LDS)."""
+    case = make_case(usx_mod, O, n_el=256, n_samples=2200, n_tx=1, n_scat=1500,
+                     grid=(4, 0.030, 0.066, 24), box=(-0.036, 0.036, 0.02, 0.07))
+    d = usx_mod.DemodConfig()
+    d.decimation = 4
+    iq = usx_mod.demodulate(case.raw, d)
+    s, txs = oracle_inputs(O, case)
+    s2 = O.Setup(**{**s.__dict__, "sampling_frequency": iq.acq.sampling_frequency,
+                    "n_samples": iq.n_samples})
+    bfc = bf_config(usx_mod, "MV", subarray=128, f_number=0.5)
+    ref, ex = port.beamform(s2, txs, np.asarray(iq.iq), case.gspec, O.Bf.of(bfc),
+                            fdem=iq.demodulation_frequency, info=True)
+    L = ex["info"][:, 1]
+    assert L.max() == 128 and ((L > 96) & (L <= 112)).any(), np.unique(L)
+    for refine in (False, True):
+        b = bf_config(usx_mod, "MV", subarray=128, f_number=0.5, refine=refine)
+        got = np.asarray(usx_mod.Beamformer(b).beamform(iq, case.grid).data)
+        assert rel_l2(got, ref) < TOL_MV, (refine, rel_l2(got, ref))
 
 
 def _pc(usx, bfc):

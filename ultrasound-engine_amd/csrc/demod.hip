@@ -10,7 +10,8 @@
 // the output is bitwise the reference's.
 //
 // demod_tile_kernel (33 taps, decimation 1, 2 or 4: the configurations'
-// filters): one wavefront per 64 consecutive outputs of one trace. It mixes
+// filters): one wavefront per 64 consecutive outputs of one trace, four per
+// workgroup (so the CUs' workgroup slots do not cap residency). It mixes
 // the input window those outputs read (64 dec + 32 samples, coalesced) into
 // LDS split by decimation phase -- sample i of the window at
 // [i mod dec][i / dec] -- so that for each tap j all 64 lanes read
@@ -38,18 +39,26 @@ __device__ __forceinline__ float2 fir_tap(float2 acc, float2 m, float h) {
 }  // namespace
 
 template <int DEC, int NT>
-__global__ __launch_bounds__(64) void demod_tile_kernel(int n_in, int n_out,
-                                                        const float* __restrict__ taps,
-                                                        const float2* __restrict__ rot,
-                                                        const float* __restrict__ rf,
-                                                        float2* __restrict__ iq) {
+__global__ __launch_bounds__(256) void demod_tile_kernel(int n_in, int n_out, int n_tiles,
+                                                         int tiles_per_trace,
+                                                         const float* __restrict__ taps,
+                                                         const float2* __restrict__ rot,
+                                                         const float* __restrict__ rf,
+                                                         float2* __restrict__ iq) {
   constexpr int GD = NT / 2;
   constexpr int W = 63 * DEC + NT;            // input samples the tile reads
   constexpr int S = (W + DEC - 1) / DEC;      // entries per phase row
-  __shared__ float2 win[DEC * S];
-  const int lane = threadIdx.x;
-  const int k0 = blockIdx.y * 64;
-  const size_t tr = blockIdx.x;
+  __shared__ float2 wins[4][DEC * S];
+  // four tiles per workgroup, one per wavefront (tiles numbered trace-major)
+  const int wv = threadIdx.x >> 6;
+  // a wavefront past the last tile repeats it and stores nothing (every
+  // wavefront reaches the barrier)
+  const int tile = min(blockIdx.x * 4 + wv, n_tiles - 1);
+  const bool store = blockIdx.x * 4 + wv < n_tiles;
+  float2* win = wins[wv];
+  const int lane = threadIdx.x & 63;
+  const size_t tr = static_cast<size_t>(tile / tiles_per_trace);
+  const int k0 = (tile - static_cast<int>(tr) * tiles_per_trace) * 64;
   const int n_lo = k0 * DEC + GD - (NT - 1);  // input index of window sample 0
   const float* x = rf + tr * n_in;
   // every load of the window issued before any is used: one memory round trip
@@ -98,7 +107,7 @@ __global__ __launch_bounds__(64) void demod_tile_kernel(int n_in, int n_out,
       }
     }
   }
-  if (k < n_out) iq[tr * n_out + k] = make_float2(acc.x * 2.0f, acc.y * 2.0f);
+  if (store && k < n_out) iq[tr * n_out + k] = make_float2(acc.x * 2.0f, acc.y * 2.0f);
 }
 
 __global__ __launch_bounds__(256) void demod_lds_kernel(int n_in, int n_out, int dec, int n_taps,
@@ -155,9 +164,12 @@ __global__ __launch_bounds__(256) void demod_global_kernel(int n_traces, int n_i
 void launch_demod(int n_traces, int n_in, int n_out, int dec, int n_taps, const float* taps,
                   const float2* rotator, const float* rf, float2* iq, hipStream_t st) {
   if (n_taps == 33 && (dec == 1 || dec == 2 || dec == 4)) {
-    const dim3 grid(n_traces, (n_out + 63) / 64);
-#define USX_DEMOD(D) \
-  hipLaunchKernelGGL((demod_tile_kernel<D, 33>), grid, dim3(64), 0, st, n_in, n_out, taps, rotator, rf, iq)
+    const int per = (n_out + 63) / 64;
+    const int n_tiles = n_traces * per;
+    const dim3 grid((n_tiles + 3) / 4);
+#define USX_DEMOD(D)                                                                          \
+  hipLaunchKernelGGL((demod_tile_kernel<D, 33>), grid, dim3(256), 0, st, n_in, n_out, n_tiles, \
+                     per, taps, rotator, rf, iq)
     if (dec == 1) USX_DEMOD(1);
     else if (dec == 2) USX_DEMOD(2);
     else USX_DEMOD(4);

@@ -76,7 +76,7 @@ int mv_max_bucket_p(int n_ch, int subarray);
 // Dynamic LDS per wavefront of the general MV kernel for bucket P (refine:
 // the full matrix, else its packed lower triangle).
 size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P, bool refine);
-constexpr int kMvMaxP = 96;
+constexpr int kMvMaxP = 128;
 
 // apod.hip: learned apodization MLP (256 -> 3 x 256 GELU -> 256, bf16 MFMA).
 constexpr int kApodFeatures = 256;

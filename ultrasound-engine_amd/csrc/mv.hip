@@ -216,6 +216,14 @@ __device__ __forceinline__ double group_sum_d(double v) {
   for (int o = 32 / G; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+template <int G>
+__device__ __forceinline__ double group_bcast_d(double v, int k, int lane) {
+  const long long u = __double_as_longlong(v);
+  const float lo = group_bcast<G>(__int_as_float(static_cast<int>(u)), k, lane);
+  const float hi = group_bcast<G>(__int_as_float(static_cast<int>(u >> 32)), k, lane);
+  return __longlong_as_double((static_cast<long long>(__float_as_int(hi)) << 32) |
+                              static_cast<unsigned int>(__float_as_int(lo)));
+}
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
@@ -1081,7 +1089,7 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
   }
 }
 
-template <int P, int NR, int RS, int G>
+template <int P, int NR, int RS, int G, bool REF>
 __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
                                                     const uint32_t* __restrict__ mask,
                                                     float2* __restrict__ frame, float* acc,
@@ -1272,6 +1280,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       b2r = ((pr[0] + pr[1]) + (pr[2] + pr[3])) / static_cast<float>(ns);
       b2i = ((pi[0] + pi[1]) + (pi[2] + pi[3])) / static_cast<float>(ns);
     }
+    const float avr = b2r, avi = b2i;  // the subarray average, kept for the refined weights
     __syncthreads();
 
     // ---- Cholesky, left-looking in blocks of CB columns, with both forward
@@ -1287,6 +1296,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     // pixels per wavefront) or on the VALU (two pixels per wavefront)
     constexpr bool kMfmaUpdate = G != 2;
     float b1r = sub < L ? 1.0f : 0.0f, b1i = 0.0f;
+    float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
     bool ok = true;  // this group's pivots so far are positive
     for (int j0 = 0; j0 < Lw; j0 += CB) {
       // Lanes outside [j0, L) keep zero partial sums, so their factor entries
@@ -1353,6 +1363,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
         const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
         const bool piv = col && sub == j, below = col && sub > j;
+        if (REF) rjj = piv ? rj : rjj;
         // rows not below the pivot take l = 0: their entries of this column are
         // never read (rows above j are not stored, and the factor's diagonal is
         // not read back), so b -= l y leaves them as they are without a select
@@ -1377,12 +1388,98 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       if (!__any(ok)) break;  // every group's factorisation failed
     }
     const bool on = sub < L;
-    const float dn = group_sum<G>(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);   // |y1|^2 = sum w
-    const float nr = group_sum<G>(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);   // y1^H y2 = w^H avg
-    const float ni = group_sum<G>(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
-    const bool das = !ok || !(dn >= 1e-30f);
     const float nf = static_cast<float>(na);
-    float2 out = make_float2(nr / dn * nf, ni / dn * nf);
+    bool das;
+    float2 out;
+    if constexpr (!REF) {
+      const float dn = group_sum<G>(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);   // |y1|^2 = sum w
+      const float nr = group_sum<G>(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);   // y1^H y2 = w^H avg
+      const float ni = group_sum<G>(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
+      das = !ok || !(dn >= 1e-30f);
+      out = make_float2(nr / dn * nf, ni / dn * nf);
+    } else {
+      // ---- one step of iterative refinement ---------------------------------
+      // w = L^-H y1 (back substitution), the residual r = 1 - R w in FP64, the
+      // correction c = R^-1 r with the same FP32 factor, w += c; then the
+      // reference's (w^H avg) / (sum w) * n (beamformer.cpp:198-213). R is not
+      // kept: R w = (1/N) X (X^H w) + eps w over the N = nrow * n_sub staged
+      // snapshots x_{r,s}[a] = rows_r[s + a], two FP64 correlations.
+      // Column-oriented substitutions, lane i holding z_i: the pivot lane's
+      // z_j / L_jj is broadcast and the rows it reaches are updated.
+      auto forward = [&](float& zr, float& zi) {  // z <- L^-1 z
+        for (int j = 0; j < Lw; ++j) {
+          const bool col = j < L;
+          const float yr = group_bcast<G>(zr * rjj, j, lane), yi = group_bcast<G>(zi * rjj, j, lane);
+          const bool below = col && sub > j && sub < L;
+          const float2 l = A[below ? at(sub, j) : 0];  // L(i, j)
+          const float nr_ = fmaf(-l.x, yr, fmaf(l.y, yi, zr)), ni_ = fmaf(-l.x, yi, fmaf(-l.y, yr, zi));
+          if (col && sub == j) {
+            zr = yr;
+            zi = yi;
+          } else if (below) {
+            zr = nr_;
+            zi = ni_;
+          }
+        }
+      };
+      auto backward = [&](float& zr, float& zi) {  // z <- L^-H z
+        for (int j = Lw - 1; j >= 0; --j) {
+          const bool col = j < L;
+          const float xr = group_bcast<G>(zr * rjj, j, lane), xi = group_bcast<G>(zi * rjj, j, lane);
+          const bool above = col && sub < j;
+          const float2 l = A[above ? at(j, sub) : 0];  // L(j, i): z_i -= conj(L(j, i)) x_j
+          const float nr_ = fmaf(-l.x, xr, fmaf(-l.y, xi, zr)), ni_ = fmaf(-l.x, xi, fmaf(l.y, xr, zi));
+          if (col && sub == j) {
+            zr = xr;
+            zi = xi;
+          } else if (above) {
+            zr = nr_;
+            zi = ni_;
+          }
+        }
+      };
+      float wr = b1r, wi = b1i;
+      backward(wr, wi);
+      double xr = 0.0, xi = 0.0;  // (X X^H w)_a, a = sub
+      for (int r = 0; r < NR; ++r) {
+        const float2* x = rows + r * RS;
+        for (int s0 = 0; s0 < nsw; s0 += LP) {
+          const int s = s0 + sub;
+          double tr = 0.0, ti = 0.0;  // t_s = x_s^H w
+          for (int k = 0; k < Lw; ++k) {
+            const float wkr = group_bcast<G>(wr, k, lane), wki = group_bcast<G>(wi, k, lane);
+            if (k < L && s < ns) {
+              const float2 v = x[s + k];
+              tr += static_cast<double>(v.x) * wkr + static_cast<double>(v.y) * wki;
+              ti += static_cast<double>(v.x) * wki - static_cast<double>(v.y) * wkr;
+            }
+          }
+          const int qn = min(LP, nsw - s0);  // snapshots of this chunk (wavefront-uniform)
+          for (int q = 0; q < qn; ++q) {
+            const double tqr = group_bcast_d<G>(tr, q, lane), tqi = group_bcast_d<G>(ti, q, lane);
+            if (s0 + q < ns && on) {
+              const float2 v = x[s0 + q + sub];
+              xr += static_cast<double>(v.x) * tqr - static_cast<double>(v.y) * tqi;
+              xi += static_cast<double>(v.x) * tqi + static_cast<double>(v.y) * tqr;
+            }
+          }
+        }
+      }
+      const double invn = 1.0 / static_cast<double>(nrow * ns);
+      float cr = on ? static_cast<float>(1.0 - (invn * xr + static_cast<double>(eps) * wr)) : 0.0f;
+      float ci = on ? static_cast<float>(-(invn * xi + static_cast<double>(eps) * wi)) : 0.0f;
+      forward(cr, ci);
+      backward(cr, ci);
+      wr += cr;
+      wi += ci;
+      const float sr_ = group_sum<G>(on ? wr : 0.0f), si_ = group_sum<G>(on ? wi : 0.0f);     // sum w
+      const float nr = group_sum<G>(on ? fmaf(wr, avr, wi * avi) : 0.0f);                      // w^H avg
+      const float ni = group_sum<G>(on ? fmaf(wr, avi, -(wi * avr)) : 0.0f);
+      const float d2 = fmaf(sr_, sr_, si_ * si_);
+      das = !ok || !(sqrtf(d2) >= 1e-30f);
+      // (w^H avg) / (sum w), complex, times n
+      out = make_float2((nr * sr_ + ni * si_) / d2 * nf, (ni * sr_ - nr * si_) / d2 * nf);
+    }
     // DAS fallback and coherence factor over the group's active centre vector
     // (the staged centre row: yc[act[a]] for a < na)
     const float2* cen = rows + sg.crow * RS;
@@ -1416,7 +1513,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
 template <int P>
 static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
                           float* acc, const int* counts, const int* lists, hipStream_t st) {
-  const bool fast = P <= 64 && M.precision == 0 && !M.refine && M.weights == nullptr &&
+  const bool fast = P <= 64 && M.precision == 0 && M.weights == nullptr &&
                     M.range_avg >= 0 && M.range_avg <= 2 && mv_fast_row_stride(M.n_ch, M.subarray, P) > 0;
   const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
                            : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P, M.refine != 0);
@@ -1449,19 +1546,22 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
       constexpr int G = P <= 16 ? 4 : (P <= 32 ? 2 : 1);  // mv_fast_groups(P)
       constexpr int RT = mv_tight_stride(P) < 128 ? mv_tight_stride(P) : 128;
       const int rs = mv_fast_row_stride(M.n_ch, M.subarray, P);
-      if (rs == RT && RT < 128) {
-        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, RT, G>));
-        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, RT, G>));
-        else USX_LAUNCH((mv_solve_fast<P, 5, RT, G>));
-      } else if (rs == 128) {
-        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, 128, G>));
-        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, 128, G>));
-        else USX_LAUNCH((mv_solve_fast<P, 5, 128, G>));
-      } else {
-        if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, 256, G>));
-        else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, 256, G>));
-        else USX_LAUNCH((mv_solve_fast<P, 5, 256, G>));
-      }
+#define USX_FAST_NR(RSV, REFV)                                                        \
+  do {                                                                                \
+    if (M.range_avg == 0) USX_LAUNCH((mv_solve_fast<P, 1, RSV, G, REFV>));            \
+    else if (M.range_avg == 1) USX_LAUNCH((mv_solve_fast<P, 3, RSV, G, REFV>));       \
+    else USX_LAUNCH((mv_solve_fast<P, 5, RSV, G, REFV>));                             \
+  } while (0)
+#define USX_FAST(REFV)                                                                \
+  do {                                                                                \
+    if (rs == RT && RT < 128) USX_FAST_NR(RT, REFV);                                  \
+    else if (rs == 128) USX_FAST_NR(128, REFV);                                       \
+    else USX_FAST_NR(256, REFV);                                                      \
+  } while (0)
+      if (M.refine) USX_FAST(true);
+      else USX_FAST(false);
+#undef USX_FAST
+#undef USX_FAST_NR
       return;
     }
   }
@@ -1489,6 +1589,8 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
   if (pmax >= 64) launch_bucket<64>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 80) launch_bucket<80>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 96) launch_bucket<96>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 112) launch_bucket<112>(M, Y, mask, frame, acc, counts, lists, st);
+  if (pmax >= 128) launch_bucket<128>(M, Y, mask, frame, acc, counts, lists, st);
 }
 
 }  // namespace usxd
