@@ -303,11 +303,12 @@ __device__ __forceinline__ bool gathers_inside(const BfArgs& A, const Pt& p, boo
 // and so is the event of slot s unless SEL (Select compounding: the slot's
 // own event): each slot's trace then starts at a scalar address and only the
 // sample index is per lane.
-template <int NE, bool IQ, bool SEL, bool CHK = true>
+template <int NE, bool IQ, bool SEL, bool CHK = true, bool T0Z = false>
 __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
                                                 const TxSlot (&sl)[NE], int nsl, int ch, float a,
                                                 int ev_first) {
-  // (nsl <= NE slots are live; !CHK: gathers_inside holds for this wavefront)
+  // (nsl <= NE slots are live; !CHK: gathers_inside holds for this wavefront;
+  // T0Z: t0 == 0, where (tau + trx) - t0 is tau + trx exactly)
   const float trx = CHK ? rx_time(A.ex[ch], p, A.c, A.inv_c) : rx_time_normal(A.ex[ch], p, A.c, A.inv_c);
   const float nm1 = static_cast<float>(A.n_s - 1);
   float2 acc = make_float2(0.0f, 0.0f);
@@ -327,7 +328,7 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
         const int s = s0 + q;
         if (s >= NE) break;
         const bool on = s < nsl;
-        const float idx = (sl[on ? s : 0].tau + trx - A.t0) * A.fs;
+        const float idx = T0Z ? (sl[on ? s : 0].tau + trx) * A.fs : (sl[on ? s : 0].tau + trx - A.t0) * A.fs;
         const int ev = SEL ? sl[on ? s : 0].ev : ev_first + (on ? s : 0);
         const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
         if (SEL) {
@@ -393,15 +394,16 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
   const RxAperture ap = rx_aperture(A, p, valid);
   // the channel loop twice: without bounds tests when every gather of the
   // wavefront is inside the traces, with them otherwise
-  auto channels = [&](auto checked) {
+  auto channels = [&](auto checked, auto t0_zero) {
     constexpr bool CHK = decltype(checked)::value;
+    constexpr bool T0Z = decltype(t0_zero)::value;
   for (int ch = lo + w; ch <= hi; ch += 4) {
     bool on;
     const float a = rx_apod(A, ap, ch, p, &on);
     if (!on) continue;
     ++na;
     const float2 y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                         : channel_value<NE, IQ, false, CHK>(A, data, p, sl, nsl, ch, a, ev_first);
+                         : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, a, ev_first);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -424,10 +426,14 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
     }
   }
   };
-  if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi))
-    channels(std::false_type{});
-  else
-    channels(std::true_type{});
+  if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi)) {
+    if (A.t0 == 0.0f)
+      channels(std::false_type{}, std::true_type{});
+    else
+      channels(std::false_type{}, std::false_type{});
+  } else {
+    channels(std::true_type{}, std::false_type{});
+  }
   const float mine[NPART] = {das.x, das.y, ds.x, ds.y, dq.x, dq.y, coh.x, coh.y, inc,
                              static_cast<float>(na)};
 #pragma unroll
@@ -511,8 +517,9 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   // wavefront is inside the traces, with them otherwise (gathers_inside is
   // wavefront-uniform and the four wavefronts share pixels and channel range,
   // so all take the same copy and meet the same barriers)
-  auto channels = [&](auto checked) {
+  auto channels = [&](auto checked, auto t0_zero) {
     constexpr bool CHK = decltype(checked)::value;
+    constexpr bool T0Z = decltype(t0_zero)::value;
     for (int c0 = lo & ~31; c0 <= hi; c0 += 32) {
       // each wave: channels c0 + w + 4k of this lane's pixel; their mask bits
       // (bit cc of word c0 / 32) gathered in a register, one LDS OR per block
@@ -530,7 +537,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
           if (on) {
             mybits |= 1u << cc;
             y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                    : channel_value<NE, IQ, false, CHK>(A, data, p, sl, nsl, ch, a, ev_first);
+                    : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, a, ev_first);
           }
         }
         tile[lane][cc] = y;
@@ -558,10 +565,14 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
     }
   };
   if (lo <= hi) {
-    if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi))
-      channels(std::false_type{});
-    else
-      channels(std::true_type{});
+    if (!sel && gathers_inside<NE>(A, p, valid, sl, nsl, lo, hi)) {
+      if (A.t0 == 0.0f)
+        channels(std::false_type{}, std::true_type{});
+      else
+        channels(std::false_type{}, std::false_type{});
+    } else {
+      channels(std::true_type{}, std::false_type{});
+    }
   }
   __syncthreads();
   if (valid && w == 0) {

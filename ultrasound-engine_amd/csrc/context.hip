@@ -484,6 +484,7 @@ void enqueue_beamform(usx_ctx* c, const usx_bf_desc* b, const BfArgs& A, const S
   M.refine = b->mv_refine;
   M.occupancy = T.mv_occupancy;
   M.n_cu = c->n_cu;
+  M.n_xcd = c->n_xcd;
   if (comb == 3 && !c->apod_loaded)
     fail(c, USX_ERR_INVALID, "learned apodization: no network loaded (usx_apod_set_params)");
   auto mv = [&](float* acc) {

@@ -67,6 +67,7 @@ struct MvArgs {
   int* status;
   int occupancy;   // persistent workgroups per CU per bucket (0: as many as LDS allows, <= 32)
   int n_cu;        // compute units of the device (sizes the persistent grids)
+  int n_xcd;       // XCDs of the device (the persistent grids give each a contiguous run of pixels)
 };
 // counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,

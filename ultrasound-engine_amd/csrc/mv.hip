@@ -216,6 +216,21 @@ __device__ __forceinline__ double group_sum_d(double v) {
   for (int o = 32 / G; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// This workgroup's slot in each round of a persistent grid. Workgroups are
+// dealt to the XCDs round robin (b runs on XCD b mod X), so the slots are
+// handed out in runs of Q per XCD: neighbouring pixels of a line, which stage
+// each other's depth rows, then mostly meet in one XCD's L2 instead of each
+// fetching those rows from HBM. Runs are short because list order follows
+// depth and the work per pixel grows with it: one long run per XCD and round
+// would give one XCD the deepest pixels every time (measured 3 % slower).
+__device__ __forceinline__ int xcd_slot(int X) {
+  constexpr int Q = 8;
+  const int g = gridDim.x, b = blockIdx.x;
+  if (X <= 1 || g % (X * Q) != 0) return b;
+  const int x = b % X, r = b / X;
+  return (r / Q) * (Q * X) + x * Q + r % Q;
+}
+
 template <int G>
 __device__ __forceinline__ double group_bcast_d(double v, int k, int lane) {
   const long long u = __double_as_longlong(v);
@@ -505,7 +520,7 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
   float2* rv = dg + P;  // scratch vector
 
   const int count = counts[BUCKET];
-  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+  for (int item = xcd_slot(M.n_xcd); item < count; item += gridDim.x) {
     const int pix = lists[static_cast<size_t>(BUCKET) * n_pix + item];
     __syncthreads();  // LDS of the previous pixel is free
 
@@ -1118,7 +1133,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
 
   const int count = counts[BUCKET];
   const int* list = lists + static_cast<size_t>(BUCKET) * n_pix;
-  for (int it = blockIdx.x * G; it < count; it += gridDim.x * G) {
+  for (int it = xcd_slot(M.n_xcd) * G; it < count; it += gridDim.x * G) {
     // a group past the end of the list repeats the wavefront's first pixel
     // and stores nothing
     const bool valid = it + g < count;
