@@ -1535,9 +1535,14 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
   const int n_pix = M.n0 * M.n1;
   // Persistent grid: as many single-wavefront workgroups per CU as can be
-  // resident at once -- bounded by LDS (160 KB / smem), by VGPRs (4 SIMDs x
-  // min(8, 512 / VGPRs per wave), read from the kernel's attributes once)
-  // and by the tuned occupancy when one is given.
+  // resident at once -- what the runtime reports for this kernel and LDS
+  // size, bounded by LDS (160 KB / smem) and by VGPRs (4 SIMDs x min(8, 512 /
+  // VGPRs per wave)) -- and by the tuned occupancy when one is given. The
+  // pixels are dealt to the workgroups in equal shares, so the grid is sized
+  // to fill every CU exactly: a CU with room to spare may be handed more
+  // workgroups than its neighbours (measured 704 -> 804 us at P = 48 when 12
+  // would fit but 11 per CU were launched), and one workgroup more than fits
+  // waits for another to end.
   const int cap = M.occupancy > 0 ? M.occupancy : 32;
   const int lds_cap = static_cast<int>(160 * 1024 / smem);
 #define USX_LAUNCH(KERNEL, ...)                                                                   \
@@ -1551,7 +1556,11 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
         return 16;                                                                               \
       return 4 * std::max(1, std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8)));                    \
     }();                                                                                         \
-    const int per_cu = std::max(1, std::min(std::min(cap, lds_cap), vgpr_cap));                  \
+    int resident = 0;                                                                            \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(KERNEL), \
+                                                     64, smem) != hipSuccess || resident <= 0)   \
+      resident = 32;                                                                             \
+    const int per_cu = std::max(1, std::min(std::min(cap, resident), std::min(lds_cap, vgpr_cap))); \
     const int grid = std::min(n_pix, M.n_cu * per_cu);                                           \
     hipLaunchKernelGGL(KERNEL, dim3(grid), dim3(64), smem, st, M, Y, mask, frame, acc, counts,   \
                        lists __VA_OPT__(, ) __VA_ARGS__);                                        \
