@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   }
   const bool take = valid && na >= 4;
   const int L = min(max(M.subarray, 2), max(2, na / 2));
-  const int b = take ? (L + 15) / 16 - 1 : -1;
+  const int b = take ? mv_bucket(L, mv_fast_config(M)) : -1;
   const int slot = bucket_slot(counts, b, take);
   if (take) lists[static_cast<size_t>(b) * n_pix + slot] = pix;
 }
@@ -1294,30 +1294,38 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       }
       b2r = ((pr[0] + pr[1]) + (pr[2] + pr[3])) / static_cast<float>(ns);
       b2i = ((pi[0] + pi[1]) + (pi[2] + pi[3])) / static_cast<float>(ns);
+      // The right-hand sides 1 and avg enter as two rows below the matrix:
+      // the Cholesky factor of [R B; B^H *], B = [1 avg], has rows L and
+      // L + 1 equal to conj(L^-1 1) and conj(L^-1 avg), so the column loop's
+      // row-per-lane work does both forward substitutions (rows L + 1 < P:
+      // the buckets hold L <= P - 2).
+      A[at(L, sub)] = make_float2(1.0f, 0.0f);
+      A[at(L + 1, sub)] = make_float2(b2r, -b2i);
     }
     const float avr = b2r, avi = b2i;  // the subarray average, kept for the refined weights
     __syncthreads();
 
-    // ---- Cholesky, left-looking in blocks of CB columns, with both forward
-    // substitutions. For the block [j0, j0 + CB) the terms of the finished
-    // columns, s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k)), are formed
-    // on MFMA in the triangle (block_update_mfma), or by lane i for its own
-    // row (one read of its row per k feeds CB accumulators, k increasing);
-    // lane i then finishes its row of the block column by column with the
-    // in-block terms k = j0 .. j0+c-1 (taken by readlane from the registers of
-    // lane j0+c), in increasing k as the reference (beamformer.cpp:48-63).
+    // ---- Cholesky of the bordered matrix, left-looking in blocks of CB
+    // columns over the L columns of R and the L + 2 rows. For the block
+    // [j0, j0 + CB) the terms of the finished columns,
+    // s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k)), are formed on MFMA
+    // in the triangle (block_update_mfma), or by lane i for its own row (one
+    // read of its row per k feeds CB accumulators, k increasing); lane i then
+    // finishes its row of the block column by column with the in-block terms
+    // k = j0 .. j0+c-1 (taken by readlane from the registers of lane j0+c), in
+    // increasing k as the reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
     // the finished columns' terms on MFMA (measured faster for one and four
     // pixels per wavefront) or on the VALU (two pixels per wavefront)
     constexpr bool kMfmaUpdate = G != 2;
-    float b1r = sub < L ? 1.0f : 0.0f, b1i = 0.0f;
     float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
     bool ok = true;  // this group's pivots so far are positive
+    const int L2 = L + 2;  // rows of the bordered matrix
     for (int j0 = 0; j0 < Lw; j0 += CB) {
-      // Lanes outside [j0, L) keep zero partial sums, so their factor entries
-      // are 0 and the substitution updates below leave their b unchanged.
+      // Lanes outside [j0, L + 2) keep zero partial sums, so their factor
+      // entries are 0.
       if (kMfmaUpdate && j0 > 0) {
-        block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, Lw, L);
+        block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, Lw + 2, L);
         __syncthreads();
       }
       float sr[CB], si[CB], lr[CB], li[CB];
@@ -1329,7 +1337,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       {
         // branch-free: every lane reads (its own entry, or entry 0) and keeps
         // the value where its row reaches the column
-        const bool rowon = sub >= j0 && sub < L;
+        const bool rowon = sub >= j0 && sub < L2;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
           const bool on = rowon && j0 + c <= sub;
@@ -1338,7 +1346,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           si[c] = on ? v.y : 0.0f;
         }
       }
-      if (sub >= j0 && sub < L) {
+      if (sub >= j0 && sub < L2) {
         if constexpr (!kMfmaUpdate) {
 #pragma unroll 2
           for (int k = 0; k < j0; ++k) {
@@ -1355,9 +1363,9 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           }
         }
       }
-      // Column by column, branch-free (selects): the pivot row takes L_jj and
-      // y_j, rows below take L_ij and b_i -= L_ij y_j, rows above keep b. A
-      // group whose pixel has no column j (j >= L) changes nothing.
+      // Column by column, branch-free (selects): rows below the pivot take
+      // L_ij (rows L and L + 1 included: the forward substitutions). A group
+      // whose pixel has no column j (j >= L) changes nothing.
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         const int j = j0 + c;
@@ -1373,28 +1381,17 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         ok = ok && (!col || d > 0.0f);
         // 1 / L_jj in one instruction (v_rsq_f32, ~1 ulp); L_jj itself is
         // never needed (the pivot row stores 0 in its place). A group whose
-        // pixel has no column j takes 0, so its y stay finite and its b as they are.
+        // pixel has no column j takes 0.
         const float rj = col ? __builtin_amdgcn_rsqf(d) : 0.0f;
-        const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
-        const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
         const bool piv = col && sub == j, below = col && sub > j;
         if (REF) rjj = piv ? rj : rjj;
         // rows not below the pivot take l = 0: their entries of this column are
         // never read (rows above j are not stored, and the factor's diagonal is
-        // not read back), so b -= l y leaves them as they are without a select
-        const float ar = below ? sr[c] * rj : 0.0f, ai = below ? si[c] * rj : 0.0f;
-        lr[c] = ar;
-        li[c] = ai;
-        const float n1r = fmaf(-ar, y1r, fmaf(ai, y1i, b1r));  // b -= l y
-        const float n1i = fmaf(-ar, y1i, fmaf(-ai, y1r, b1i));
-        const float n2r = fmaf(-ar, y2r, fmaf(ai, y2i, b2r));
-        const float n2i = fmaf(-ar, y2i, fmaf(-ai, y2r, b2i));
-        b1r = piv ? y1r : n1r;
-        b1i = piv ? y1i : n1i;
-        b2r = piv ? y2r : n2r;
-        b2i = piv ? y2i : n2i;
+        // not read back)
+        lr[c] = below ? sr[c] * rj : 0.0f;
+        li[c] = below ? si[c] * rj : 0.0f;
       }
-      if (sub >= j0 && sub < L) {
+      if (sub >= j0 && sub < L2) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
           if (j0 + c < L && j0 + c <= sub) A[at(sub, j0 + c)] = make_float2(lr[c], li[c]);
@@ -1404,12 +1401,16 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     }
     const bool on = sub < L;
     const float nf = static_cast<float>(na);
+    // rows L and L + 1 of the factor: e1 = conj(y1), e2 = conj(y2) with
+    // y1 = L^-1 1 and y2 = L^-1 avg
+    const float2 e1 = on ? A[at(L, sub)] : make_float2(0.0f, 0.0f);
+    const float2 e2 = on ? A[at(L + 1, sub)] : make_float2(0.0f, 0.0f);
     bool das;
     float2 out;
     if constexpr (!REF) {
-      const float dn = group_sum<G>(on ? fmaf(b1r, b1r, b1i * b1i) : 0.0f);   // |y1|^2 = sum w
-      const float nr = group_sum<G>(on ? fmaf(b1r, b2r, b1i * b2i) : 0.0f);   // y1^H y2 = w^H avg
-      const float ni = group_sum<G>(on ? fmaf(b1r, b2i, -(b1i * b2r)) : 0.0f);
+      const float dn = group_sum<G>(fmaf(e1.x, e1.x, e1.y * e1.y));     // |y1|^2 = sum w
+      const float nr = group_sum<G>(fmaf(e1.x, e2.x, e1.y * e2.y));     // y1^H y2 = w^H avg
+      const float ni = group_sum<G>(fmaf(e1.x, -e2.y, e1.y * e2.x));
       das = !ok || !(dn >= 1e-30f);
       out = make_float2(nr / dn * nf, ni / dn * nf);
     } else {
@@ -1453,7 +1454,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           }
         }
       };
-      float wr = b1r, wi = b1i;
+      float wr = e1.x, wi = -e1.y;
       backward(wr, wi);
       double xr = 0.0, xi = 0.0;  // (X X^H w)_a, a = sub
       for (int r = 0; r < NR; ++r) {
@@ -1528,8 +1529,8 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
 template <int P>
 static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
                           float* acc, const int* counts, const int* lists, hipStream_t st) {
-  const bool fast = P <= 64 && M.precision == 0 && M.weights == nullptr &&
-                    M.range_avg >= 0 && M.range_avg <= 2 && mv_fast_row_stride(M.n_ch, M.subarray, P) > 0;
+  // (mv_classify put pixels of L <= P - 2 here when this holds, else L <= P)
+  const bool fast = P <= 64 && mv_fast_config(M);
   const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
                            : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P, M.refine != 0);
   const int na_max = M.subarray <= P ? M.n_ch : std::min(M.n_ch, 2 * P + 1);
@@ -1595,9 +1596,9 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
 #undef USX_LAUNCH
 }
 
-int mv_max_bucket_p(int n_ch, int subarray) {
+int mv_max_bucket_p(int n_ch, int subarray, bool fast) {
   const int lmax = std::min(std::max(subarray, 2), std::max(2, n_ch / 2));
-  return (lmax + 15) / 16 * 16;
+  return 16 * (mv_bucket(lmax, fast) + 1);
 }
 
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
@@ -1606,7 +1607,7 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
   hipLaunchKernelGGL(mv_zero_counts, dim3(1), dim3(64), 0, st, counts);
   hipLaunchKernelGGL(mv_classify, dim3((n_pix + 255) / 256), dim3(256), 0, st, M, Y, mask, frame,
                      acc, counts, lists);
-  const int pmax = mv_max_bucket_p(M.n_ch, M.subarray);
+  const int pmax = mv_max_bucket_p(M.n_ch, M.subarray, mv_fast_config(M));
   if (pmax >= 16) launch_bucket<16>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 32) launch_bucket<32>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 48) launch_bucket<48>(M, Y, mask, frame, acc, counts, lists, st);
