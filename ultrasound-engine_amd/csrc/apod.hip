@@ -26,6 +26,7 @@
 // (g, j), only on A and B using the same map, which they do by construction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -44,9 +45,8 @@ constexpr int kMT = kF / 16;         // 16 output tiles
 constexpr int kKB = kF / 32;         // 8 K blocks
 constexpr int kLayerBf16 = kF * kF;  // swizzled bf16 per layer
 constexpr int kNT = 1;               // 16-sample tiles per wavefront (training)
-constexpr int kNTi = 1;              // 16-sample tiles per wavefront (inference)
-constexpr int kInferWaves = 8;       // wavefronts per inference workgroup
-constexpr int kStageTiles = 8;       // output tiles per LDS stage (of kMT per layer)
+constexpr int kInferWaves = 8;       // wavefronts (and 16-pixel tiles) per inference workgroup
+static_assert(2 * kInferWaves == kMT, "inference: each wavefront owns two output tiles");
 
 // FP32 -> BF16, round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint32_t f2bf(float x) {
@@ -141,62 +141,41 @@ __device__ __forceinline__ void layer(const u4* w, const float* bias, const u4 (
   }
 }
 
-// Inference stage: output tiles M0 .. M0 + MT - 1 from the LDS copy w of
-// their fragments. The MT tiles advance together one K block at a time, so
-// the wavefront keeps MT independent accumulator chains in flight, and the
-// fragments of block kb + 1 are read while block kb's MFMAs run. Each tile
-// still accumulates K blocks 0..7 in order onto its bias.
-template <int M0, int MT, int NT>
-__device__ __forceinline__ void stage_tiles(const u4* w, const float* bias,
-                                            const u4 (&act)[kKB][NT], f4 (&acc)[kMT][NT],
-                                            int lane, uint32_t kmask) {
-  const int g = lane >> 4;
-#pragma unroll
-  for (int i = 0; i < MT; ++i) {
-    const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * (M0 + i) + 4 * g);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[M0 + i][t] = f4{bb.x, bb.y, bb.z, bb.w};
-  }
-  u4 cur[MT], nxt[MT];
-  if (kmask & 1u) {
-#pragma unroll
-    for (int i = 0; i < MT; ++i) cur[i] = w[(i * kKB) * 64 + lane];
-  }
-#pragma unroll
-  for (int kb = 0; kb < kKB; ++kb) {
-    if (kb + 1 < kKB && ((kmask >> (kb + 1)) & 1u)) {
-#pragma unroll
-      for (int i = 0; i < MT; ++i) nxt[i] = w[(i * kKB + kb + 1) * 64 + lane];
-    }
-    if ((kmask >> kb) & 1u) {
-#pragma unroll
-      for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-          acc[M0 + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              as_bf8(cur[i]), as_bf8(act[kb][t]), acc[M0 + i][t], 0, 0, 0);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < MT; ++i) cur[i] = nxt[i];
-  }
-}
-
 // Network input of one pixel from its channel vector: the active channels
 // (mask bit set) rotated by conj(sum y) / |sum y| and scaled to unit mean
 // power, in BF16. Lane (g, r) of tile t holds pixel r's features for K
 // blocks kb = 0..7: channels 16 kb + 2 g + {0, 1} and 16 kb + 8 + 2 g + {0, 1}.
 // Inactive channels and invalid pixels give 0. Returns the active count.
-__device__ __forceinline__ int load_features(const float2* Y, const uint32_t* mask, int n_ch,
-                                             long pix, bool valid, int lane, u4 (&x)[kKB]) {
+struct PixelSums {
+  float sr, si, e;  // sum y (re, im) and sum |y|^2 over the active channels
+};
+// The loads of load_features, issued ahead of their use: lane (g, r)'s
+// channels of the pixel and the pixel's mask words.
+struct RawPixel {
+  float4 v[kKB][2];
+  uint32_t mw[4];
+};
+__device__ __forceinline__ void fetch_pixel(const float2* Y, const uint32_t* mask, int n_ch,
+                                            long pix, bool valid, int lane, RawPixel& R) {
   const int g = lane >> 4;
   const int nw = (n_ch + 31) / 32;
-  float4 v[kKB][2];
-  uint32_t mw[4] = {0u, 0u, 0u, 0u};
-  if (valid) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) mw[q] = mask[pix * nw + q];
+  for (int q = 0; q < 4; ++q) R.mw[q] = valid ? mask[pix * nw + q] : 0u;
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ch = 16 * kb + 8 * h + 2 * g;
+      const uint32_t bits = R.mw[ch >> 5] >> (ch & 31);
+      R.v[kb][h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (bits & 3u) R.v[kb][h] = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
+    }
   }
+}
+__device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[kKB],
+                                           PixelSums* sums) {
+  const int g = lane >> 4;
+  float4 v[kKB][2];
   float sr = 0.0f, si = 0.0f, e = 0.0f;
   int na = 0;
 #pragma unroll
@@ -204,9 +183,8 @@ __device__ __forceinline__ int load_features(const float2* Y, const uint32_t* ma
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int ch = 16 * kb + 8 * h + 2 * g;
-      const uint32_t bits = mw[ch >> 5] >> (ch & 31);
-      float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (valid && (bits & 3u)) y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
+      const uint32_t bits = R.mw[ch >> 5] >> (ch & 31);
+      float4 y = R.v[kb][h];
       if (!(bits & 1u)) y.x = y.y = 0.0f;
       if (!(bits & 2u)) y.z = y.w = 0.0f;
       v[kb][h] = y;
@@ -223,6 +201,7 @@ __device__ __forceinline__ int load_features(const float2* Y, const uint32_t* ma
     e += __shfl_xor(e, o, 64);
     na += __shfl_xor(na, o, 64);
   }
+  if (sums) *sums = PixelSums{sr, si, e};
   float cr = 1.0f, ci = 0.0f, s = 0.0f;
   if (e > 0.0f && na > 0) {
     const float mag = sqrtf(sr * sr + si * si);
@@ -244,6 +223,13 @@ __device__ __forceinline__ int load_features(const float2* Y, const uint32_t* ma
   }
   return na;
 }
+__device__ __forceinline__ int load_features(const float2* Y, const uint32_t* mask, int n_ch,
+                                             long pix, bool valid, int lane, u4 (&x)[kKB],
+                                             PixelSums* sums = nullptr) {
+  RawPixel R;
+  fetch_pixel(Y, mask, n_ch, pix, valid, lane, R);
+  return features_of(R, lane, x, sums);
+}
 
 // The next layer's B operand from output tiles 2 kb and 2 kb + 1, through GELU.
 template <int NT = kNT>
@@ -262,160 +248,241 @@ __device__ __forceinline__ void activate(const f4 (&acc)[kMT][NT], u4 (&act)[kKB
 
 }  // namespace
 
-// Weights of the inference kernel move HBM/L2 -> LDS one stage (kStageTiles
-// output tiles of a layer) at a time, into two LDS buffers used alternately,
-// by direct-to-LDS loads (global_load_lds_dwordx4: each lane's 16 bytes land
-// at the wavefront's LDS base + 16 lane, no registers): the next stage's
-// loads are issued before the current stage's MFMAs, into the other buffer,
-// and waited for (vmcnt) before the stage's one barrier.
-constexpr int kStageU4 = kStageTiles * kKB * 64;
-constexpr int kPre = kStageU4 / (64 * kInferWaves);  // 16-byte rows per thread and stage
-static_assert(kMT % kStageTiles == 0 && kStageU4 % (64 * kInferWaves) == 0, "stage shape");
-static_assert((kMT / kStageTiles) % 2 == 0, "stages per layer must alternate the buffers evenly");
+// ---------------------------------------------------------------------------
+// Inference, weight-stationary: one workgroup = kInferWaves wavefronts and
+// kInferWaves 16-pixel sample tiles (128 pixels, consecutive depths of a
+// line). Wavefront w owns output tiles 2w and 2w + 1 of every layer -- the
+// features of K block w of the next layer -- and holds their 16 weight
+// fragments in registers (64 VGPRs; the next layer's are loaded from L2 while
+// this layer runs). The activations of the 128 pixels sit in LDS in the B
+// operand layout ([tile][K block][lane], 16 bytes per lane, 64 KB a layer,
+// two buffers): each 16-byte fragment read feeds two MFMAs (both of the
+// wavefront's output tiles), and the weights never pass through LDS. After
+// GELU a wavefront's two output tiles are exactly K block w of the next
+// layer's input, one 16-byte store per sample tile. One barrier per layer.
+// Each output tile accumulates K blocks 0..7 in order onto its bias (no
+// K block is skipped: a conditional MFMA costs more VALU in accumulator
+// copies than the products it saves, and zero features add exactly
+// nothing). The last layer's outputs o_c stay in the owning wavefront, which
+// forms its channels' share of sum (1 + o_c) y_c per pixel; the shares meet
+// in LDS. The kernel is VALU-bound (GELU: two quarter-rate transcendentals
+// per activation), not bound by the MFMAs or by LDS.
+// ---------------------------------------------------------------------------
+constexpr int kActU4 = kInferWaves * kKB * 64;  // one activation buffer (u4), 64 KB
 
-typedef __attribute__((address_space(3))) void* lds_ptr;
-__device__ __forceinline__ void load_stage(const u4* wsrc, int l, int m0, int tid, u4* buf) {
-  const u4* src = wsrc + l * (kLayerBf16 / 8) + m0 * kKB * 64;
-  u4* dst = buf + (tid & ~63);  // the wavefront's base, uniform
-#pragma unroll
-  for (int q = 0; q < kPre; ++q)
-    __builtin_amdgcn_global_load_lds(src + tid + q * 64 * kInferWaves,
-                                     (lds_ptr)(dst + q * 64 * kInferWaves), 16, 0, 0);
+size_t apod_infer_smem_bytes() {
+  return sizeof(u4) * 2 * kActU4 + sizeof(float2) * kInferWaves * kInferWaves * 16;
 }
-// vmcnt(0), expcnt and lgkmcnt unconstrained: this wavefront's loads into LDS
-// have landed
-__device__ __forceinline__ void wait_stage() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// Layer l from output tile M0 on: issue the next stage (this layer's next, else
-// layer l + 1's first) into the other buffer (whose previous stage every
-// wavefront finished before the last barrier), run this stage's tiles from
-// its buffer, wait for the loads, barrier; compile-time recursion over the
-// stages so the accumulator tiles stay register-indexed.
-template <int M0>
-__device__ __forceinline__ void stage_and_run(u4* wl, const u4* wsrc, int l, const float* bias,
-                                              const u4 (&act)[kKB][kNTi], f4 (&acc)[kMT][kNTi],
-                                              int tid, int lane, uint32_t kmask) {
-  if constexpr (M0 < kMT) {
-    constexpr int B = (M0 / kStageTiles) & 1;
-    const bool next_layer = M0 + kStageTiles >= kMT;
-    const bool more = !next_layer || l + 1 < kL;
-    if (more)
-      load_stage(wsrc, next_layer ? l + 1 : l, next_layer ? 0 : M0 + kStageTiles, tid,
-                 wl + (B ^ 1) * kStageU4);
-    stage_tiles<M0, kStageTiles, kNTi>(wl + B * kStageU4, bias, act, acc, lane, kmask);
-    wait_stage();
-    __syncthreads();
-    stage_and_run<M0 + kStageTiles>(wl, wsrc, l, bias, act, acc, tid, lane, kmask);
+// Fragments (l, 2 wv + q, kb) through a buffer resource: one lane offset
+// (16 lane) and a scalar offset per fragment, so no per-lane 64-bit
+// addresses are kept across the tile loop.
+__device__ __forceinline__ void load_weights(__amdgpu_buffer_rsrc_t wrs, int l, int wv, int lane,
+                                             u4 (&w)[2][kKB]) {
+  const int base = (l * (kLayerBf16 / 8) + (2 * wv) * kKB * 64) * 16;
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int kb = 0; kb < kKB; ++kb)
+      w[q][kb] = __builtin_amdgcn_raw_buffer_load_b128(wrs, 16 * lane, base + (q * kKB + kb) * 1024, 0);
+}
+
+// One layer for the wavefront's two output tiles over the workgroup's
+// kInferWaves sample tiles, two sample tiles at a time (four accumulator
+// chains), software-pipelined: the MFMAs of a pair of sample tiles are
+// issued before the GELU (VALU, two quarter-rate transcendentals per value)
+// of the previous pair, so the matrix pipe and the VALU work together. LAST:
+// the wavefront's shares of sum (1 + o_c) y_c go to `part`, else the GELU'd
+// B fragments to `out`.
+__device__ __forceinline__ void ws_products(const u4 (&w)[2][kKB], const float4& b0,
+                                            const float4& b1, const u4* in, int t, int lane,
+                                            f4 (&acc)[2][2]) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    acc[0][u] = f4{b0.x, b0.y, b0.z, b0.w};
+    acc[1][u] = f4{b1.x, b1.y, b1.z, b1.w};
+  }
+  u4 cur[2], nxt[2];
+  cur[0] = in[(t * kKB) * 64 + lane];
+  cur[1] = in[((t + 1) * kKB) * 64 + lane];
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+    if (kb + 1 < kKB) {
+      nxt[0] = in[(t * kKB + kb + 1) * 64 + lane];
+      nxt[1] = in[((t + 1) * kKB + kb + 1) * 64 + lane];
+    }
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(w[0][kb]), as_bf8(cur[0]), acc[0][0], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(w[1][kb]), as_bf8(cur[0]), acc[1][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(w[0][kb]), as_bf8(cur[1]), acc[0][1], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(w[1][kb]), as_bf8(cur[1]), acc[1][1], 0, 0, 0);
+    cur[0] = nxt[0];
+    cur[1] = nxt[1];
   }
 }
 
-// Channels active at any pixel of the wavefront (wave-uniform).
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+template <bool LAST>
+__device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out, int wv, int lane,
+                                          const float2* Y, const uint32_t* mask, int n_ch,
+                                          long base, int n_pix, float2* part) {
+  const int g = lane >> 4, r = lane & 15;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) v |= __shfl_xor(v, o, 64);
-  return __builtin_amdgcn_readfirstlane(v);
+  for (int u = 0; u < 2; ++u) {
+    const f4 a = acc[0][u], b = acc[1][u];
+    if constexpr (!LAST) {
+      // K block wv of the next layer for sample tile t + u
+      out[((t + u) * kKB + wv) * 64 + lane] =
+          u4{pack2(gelu(a[0]), gelu(a[1])), pack2(gelu(a[2]), gelu(a[3])),
+             pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
+    } else {
+      // o of channels 16 wv + 2 g + {0, 1} (tile 2 wv) and 16 wv + 8 + 2 g +
+      // {0, 1} (tile 2 wv + 1) of pixel r of sample tile t + u
+      const long pix = base + 16 * (t + u) + r;
+      float lr = 0.0f, li = 0.0f;
+      if (pix < n_pix) {
+        const int nw = (n_ch + 31) / 32;
+        const int c0 = 16 * wv + 2 * g;
+        const uint32_t mw = mask[pix * nw + (c0 >> 5)];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int ch = c0 + 8 * h;
+          const uint32_t bits = mw >> (ch & 31);
+          if (!(bits & 3u)) continue;
+          float4 y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
+          if (!(bits & 1u)) y.x = y.y = 0.0f;
+          if (!(bits & 2u)) y.z = y.w = 0.0f;
+          const f4 o = h ? b : a;
+          const float g0r = (bits & 1u) ? 1.0f + o[0] : 0.0f, g0i = (bits & 1u) ? o[1] : 0.0f;
+          const float g1r = (bits & 2u) ? 1.0f + o[2] : 0.0f, g1i = (bits & 2u) ? o[3] : 0.0f;
+          lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
+          li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
+        }
+      }
+#pragma unroll
+      for (int o = 16; o < 64; o <<= 1) {
+        lr += __shfl_xor(lr, o, 64);
+        li += __shfl_xor(li, o, 64);
+      }
+      if (g == 0) part[(wv * kInferWaves + t + u) * 16 + r] = make_float2(lr, li);
+    }
+  }
 }
 
-// ---------------------------------------------------------------------------
-// Inference: one workgroup = kInferWaves wavefronts x 16 kNTi pixels. Each
-// layer's swizzled weights pass through LDS kStageTiles output tiles at a
-// time (double-buffered), shared by the workgroup's wavefronts. The pixels of
-// a wavefront are consecutive depths of one line, whose active apertures are
-// nested: the first layer skips the K blocks (16 channels) that none of them
-// has active, which adds exactly nothing.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * kInferWaves) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
+template <bool LAST>
+__device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bias, const u4* in,
+                                         u4* out, int wv, int lane,
+                                         const float2* Y, const uint32_t* mask, int n_ch, long base,
+                                         int n_pix, float2* part) {
+  const int g = lane >> 4;
+  const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
+  const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
+  static_assert(kInferWaves % 4 == 0, "sample tiles in pairs of pairs");
+  f4 acc0[2][2], acc1[2][2];
+  ws_products(w, b0, b1, in, 0, lane, acc0);
+#pragma unroll
+  for (int t = 2; t < kInferWaves; t += 4) {
+    ws_products(w, b0, b1, in, t, lane, acc1);
+    ws_finish<LAST>(acc0, t - 2, out, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    if (t + 2 < kInferWaves) ws_products(w, b0, b1, in, t + 2, lane, acc0);
+    ws_finish<LAST>(acc1, t, out, wv, lane, Y, mask, n_ch, base, n_pix, part);
+  }
+}
+
+// Stage this wavefront's 16 pixels (sample tile wv) from their fetched
+// channel vectors into the activation buffer. Inactive channels are zero
+// features, so the first layer's products over them add exactly nothing.
+__device__ __forceinline__ int stage_input(const RawPixel& R, int wv, int lane, u4* act,
+                                           PixelSums* ps) {
+  u4 x[kKB];
+  const int na = features_of(R, lane, x, ps);
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) act[(wv * kKB + kb) * 64 + lane] = x[kb];
+  return na;
+}
+
+// Persistent: workgroup b takes the 128-pixel tiles b, b + gridDim.x, ...
+// (one workgroup per CU: 136 KB of LDS). The next tile's channel vectors are
+// fetched while the last layer runs (into registers; the first layer's
+// weights, whose registers the last layer does not use, follow), so their
+// HBM latency hides behind the MFMAs.
+__global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
                                                          const float2* __restrict__ Y,
                                                          const uint32_t* __restrict__ mask,
                                                          float2* __restrict__ frame, float* acc_out,
                                                          int cf, int incoherent) {
-  extern __shared__ u4 wl[];  // 2 x kStageU4
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  extern __shared__ u4 smem[];
+  u4* act0 = smem;
+  u4* act1 = smem + kActU4;
+  float2* part = reinterpret_cast<float2*>(smem + 2 * kActU4);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int g = lane >> 4, r = lane & 15;
-  const long base = (static_cast<long>(blockIdx.x) * kInferWaves + wv) * (16 * kNTi);
-  const int nw = (n_ch + 31) / 32;
-  const u4* wsrc = reinterpret_cast<const u4*>(N.wf);
-  load_stage(wsrc, 0, 0, tid, wl);  // waited for before the barrier below
-  u4 act[kKB][kNTi];
-  int na[kNTi];
-  uint32_t un[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-  for (int t = 0; t < kNTi; ++t) {
-    const long pix = base + 16 * t + r;
-    u4 x[kKB];
-    na[t] = load_features(Y, mask, n_ch, pix, pix < n_pix, lane, x);
-#pragma unroll
-    for (int kb = 0; kb < kKB; ++kb) act[kb][t] = x[kb];
-    if (pix < n_pix) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) un[q] |= mask[pix * nw + q];
-    }
+  const int n_tiles = (n_pix + 16 * kInferWaves - 1) / (16 * kInferWaves);
+  // the forward fragments, 4 layers x 128 KB, as a raw buffer
+  const __amdgpu_buffer_rsrc_t wsrc =
+      __builtin_amdgcn_make_buffer_rsrc(N.wf, 0, kL * kLayerBf16 * 2, 0x00020000);
+  int tile = blockIdx.x;
+  u4 wa[2][kKB], wb[2][kKB];
+  load_weights(wsrc, 0, wv, lane, wa);
+  int na;
+  PixelSums ps;
+  {
+    const long pix = static_cast<long>(tile) * (16 * kInferWaves) + 16 * wv + r;
+    RawPixel R;
+    fetch_pixel(Y, mask, n_ch, pix, pix < n_pix, lane, R);
+    na = stage_input(R, wv, lane, act0, &ps);
   }
-  // bit kb: channels 16 kb .. 16 kb + 15 active at some pixel of the wavefront
-  uint32_t live = 0u;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t u = wave_or(un[q]);
-    live |= ((u & 0xffffu) ? 1u : 0u) << (2 * q);
-    live |= ((u >> 16) ? 1u : 0u) << (2 * q + 1);
-  }
-  wait_stage();
+  load_weights(wsrc, 1, wv, lane, wb);
   __syncthreads();
-  f4 acc[kMT][kNTi];
-  for (int l = 0; l < kL; ++l) {
-    stage_and_run<0>(wl, wsrc, l, N.bias + l * kF, act, acc, tid, lane, l == 0 ? live : 0xffu);
-    if (l + 1 < kL) activate<kNTi>(acc, act);
-  }
-  // out = sum over active channels of (1 + o_c) y_c; fewer than 4 active
-  // channels -> the DAS sum, as minimum variance does.
+  while (tile < n_tiles) {
+    const long base = static_cast<long>(tile) * (16 * kInferWaves);
+    ws_layer<false>(wa, N.bias + 0 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    load_weights(wsrc, 2, wv, lane, wa);
+    __syncthreads();
+    ws_layer<false>(wb, N.bias + 1 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    load_weights(wsrc, 3, wv, lane, wb);
+    __syncthreads();
+    ws_layer<false>(wa, N.bias + 2 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    __syncthreads();
+    // act0 is free: fetch the next tile's channel vectors during the
+    // last layer, then stage them
+    const int next = tile + gridDim.x;
+    const long npix = static_cast<long>(next) * (16 * kInferWaves) + 16 * wv + r;
+    RawPixel R;
+    fetch_pixel(Y, mask, n_ch, npix, npix < n_pix, lane, R);
+    ws_layer<true>(wb, N.bias + 3 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    PixelSums nps;
+    const int nna = stage_input(R, wv, lane, act0, &nps);
+    load_weights(wsrc, 0, wv, lane, wa);
+    load_weights(wsrc, 1, wv, lane, wb);
+    __syncthreads();
+    // out = sum over active channels of (1 + o_c) y_c (the wavefronts' shares
+    // in order); fewer than 4 active channels -> the DAS sum, as minimum
+    // variance does.
+    const long pix = base + 16 * wv + r;
+    if (pix < n_pix && g == 0) {
+      float lr = 0.0f, li = 0.0f;
 #pragma unroll
-  for (int t = 0; t < kNTi; ++t) {
-    const long pix = base + 16 * t + r;
-    const bool valid = pix < n_pix;
-    uint32_t mw[4] = {0u, 0u, 0u, 0u};
-    if (valid) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) mw[q] = mask[pix * nw + q];
+      for (int w = 0; w < kInferWaves; ++w) {
+        const float2 sp = part[(w * kInferWaves + wv) * 16 + r];
+        lr += sp.x;
+        li += sp.y;
+      }
+      float2 out = na >= 4 ? make_float2(lr, li) : make_float2(ps.sr, ps.si);
+      if (cf) {
+        const float mc2 = ps.sr * ps.sr + ps.si * ps.si;
+        const float f = (ps.e > 0.0f && na > 0) ? mc2 / (static_cast<float>(na) * ps.e) : 0.0f;
+        out.x *= f;
+        out.y *= f;
+      }
+      if (incoherent) {
+        acc_out[pix] += out.x * out.x + out.y * out.y;
+      } else {
+        frame[pix] = out;
+      }
     }
-    float lr = 0.0f, li = 0.0f, dr = 0.0f, di = 0.0f, inc = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kMT; ++m) {
-      const int ch = 8 * m + 2 * g;
-      const uint32_t bits = mw[ch >> 5] >> (ch & 31);
-      if (!(bits & 3u)) continue;
-      float4 y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
-      if (!(bits & 1u)) y.x = y.y = 0.0f;
-      if (!(bits & 2u)) y.z = y.w = 0.0f;
-      const f4 o = acc[m][t];
-      const float g0r = (bits & 1u) ? 1.0f + o[0] : 0.0f, g0i = (bits & 1u) ? o[1] : 0.0f;
-      const float g1r = (bits & 2u) ? 1.0f + o[2] : 0.0f, g1i = (bits & 2u) ? o[3] : 0.0f;
-      lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
-      li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
-      dr += y.x + y.z;
-      di += y.y + y.w;
-      inc += y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w;
-    }
-#pragma unroll
-    for (int o = 16; o < 64; o <<= 1) {
-      lr += __shfl_xor(lr, o, 64);
-      li += __shfl_xor(li, o, 64);
-      dr += __shfl_xor(dr, o, 64);
-      di += __shfl_xor(di, o, 64);
-      inc += __shfl_xor(inc, o, 64);
-    }
-    if (!valid || g != 0) continue;
-    float2 out = na[t] >= 4 ? make_float2(lr, li) : make_float2(dr, di);
-    if (cf) {
-      const float mc2 = dr * dr + di * di;
-      const float f = (inc > 0.0f && na[t] > 0) ? mc2 / (static_cast<float>(na[t]) * inc) : 0.0f;
-      out.x *= f;
-      out.y *= f;
-    }
-    if (incoherent) {
-      acc_out[pix] += out.x * out.x + out.y * out.y;
-    } else {
-      frame[pix] = out;
-    }
+    na = nna;
+    ps = nps;
+    tile = next;
   }
 }
 
@@ -759,7 +826,7 @@ __global__ __launch_bounds__(256) void apod_pack_kernel(ApodNet N) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-size_t apod_infer_smem() { return sizeof(u4) * 2 * kStageU4; }
+size_t apod_infer_smem() { return apod_infer_smem_bytes(); }
 
 void launch_apod_pack(const ApodNet& N, hipStream_t st) {
   hipLaunchKernelGGL(apod_pack_kernel, dim3((kApodParams + 255) / 256), dim3(256), 0, st, N);
@@ -767,13 +834,14 @@ void launch_apod_pack(const ApodNet& N, hipStream_t st) {
 
 void launch_apod_infer(const ApodNet& N, int n_pix, int n_ch, const float2* Y,
                        const uint32_t* mask, float2* frame, float* acc, int cf, int incoherent,
-                       hipStream_t st) {
-  const int per_block = kInferWaves * 16 * kNTi;
+                       int n_cu, hipStream_t st) {
+  const int per_block = kInferWaves * 16;
   static const hipError_t attr = hipFuncSetAttribute(
       reinterpret_cast<const void*>(apod_infer_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
       static_cast<int>(apod_infer_smem()));
   (void)attr;
-  hipLaunchKernelGGL(apod_infer_kernel, dim3((n_pix + per_block - 1) / per_block), dim3(64 * kInferWaves),
+  const int n_tiles = (n_pix + per_block - 1) / per_block;  // one persistent workgroup per CU
+  hipLaunchKernelGGL(apod_infer_kernel, dim3(std::max(1, std::min(n_tiles, n_cu))), dim3(64 * kInferWaves),
                      apod_infer_smem(), st, N, n_pix, n_ch, Y, mask, frame, acc, cf, incoherent);
 }
 

@@ -206,13 +206,14 @@ __device__ __forceinline__ RxAperture rx_aperture(const BfArgs& A, const Pt& p, 
   return ap;
 }
 // Receive weight of channel ch (*on: the reference's weight is non-zero).
+template <bool HANN>
 __device__ __forceinline__ float rx_apod(const BfArgs& A, const RxAperture& ap, int ch, const Pt& p,
                                         bool* on) {
   if (!ap.any) {
     *on = false;
     return 0.0f;
   }
-  return rx_weight(A.rx_window, A.tukey_alpha, A.ex[ch] - p.x, ap.d, ap.r, A.cos1_lo, A.cos1_hi, on);
+  return rx_weight<HANN>(A.rx_window, A.tukey_alpha, A.ex[ch] - p.x, ap.d, ap.r, A.cos1_lo, A.cos1_hi, on);
 }
 
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
@@ -365,7 +366,8 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
 // Launch bounds as bf_channels_kernel's (measured 0.181 -> 0.157 ms at
 // 256x640, 128 channels, 11 transmits).
 // ---------------------------------------------------------------------------
-template <int NE, int COMB, bool CF, bool IQ>
+// HANN: the receive window is Hann (rx_weight<true>).
+template <int NE, int COMB, bool CF, bool IQ, bool HANN>
 __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs A, const void* data, float2* frame,
                                                        int ev_first, int n_slots, int select,
                                                        int accumulate) {
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
     constexpr bool T0Z = decltype(t0_zero)::value;
   for (int ch = lo + w; ch <= hi; ch += 4) {
     bool on;
-    const float a = rx_apod(A, ap, ch, p, &on);
+    const float a = rx_apod<HANN>(A, ap, ch, p, &on);
     if (!on) continue;
     ++na;
     const float2 y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
@@ -488,7 +490,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
 // ---------------------------------------------------------------------------
 constexpr int kMaskWordsMax = 16;  // up to 512 channels
 
-template <int NE, bool IQ>
+template <int NE, bool IQ, bool HANN>
 __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfArgs A, const void* data, float2* Y,
                                                           uint32_t* mask, int ev_first,
                                                           int n_slots, int select, int first,
@@ -529,7 +531,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
         float2 y = make_float2(0.0f, 0.0f);
         if (valid && ch >= lo && ch <= hi) {
           bool on;
-          float a = rx_apod(A, ap, ch, p, &on);
+          float a = rx_apod<HANN>(A, ap, ch, p, &on);
           if (rect) {  // beamformer.cpp:385: a > 0 ? 1 : 0, so a NaN weight is inactive here
             on = on && a == a;
             a = 1.0f;
@@ -677,9 +679,15 @@ static void launch_fused_ne(const BfArgs& A, const void* data, float2* frame, in
                             int n_slots, int select, int accumulate, int comb, bool cf,
                             hipStream_t st) {
   const dim3 grid(tile_blocks(A)), block(256);
-#define USX_FUSED(C_, CF_) \
-  hipLaunchKernelGGL((bf_fused_kernel<NE, C_, CF_, IQ>), grid, block, 0, st, A, data, frame, \
-                     ev_first, n_slots, select, accumulate)
+#define USX_FUSED(C_, CF_)                                                                        \
+  do {                                                                                            \
+    if (A.rx_window == 1)                                                                         \
+      hipLaunchKernelGGL((bf_fused_kernel<NE, C_, CF_, IQ, true>), grid, block, 0, st, A, data,   \
+                         frame, ev_first, n_slots, select, accumulate);                           \
+    else                                                                                          \
+      hipLaunchKernelGGL((bf_fused_kernel<NE, C_, CF_, IQ, false>), grid, block, 0, st, A, data,  \
+                         frame, ev_first, n_slots, select, accumulate);                           \
+  } while (0)
   if (comb == 0 && !cf) USX_FUSED(0, false);
   else if (comb == 0 && cf) USX_FUSED(0, true);
   else if (!cf) USX_FUSED(1, false);
@@ -717,8 +725,12 @@ template <int NE, bool IQ>
 static void launch_channels_ne(const BfArgs& A, const void* data, float2* Y, uint32_t* mask,
                                int ev_first, int n_slots, int select, int first, int rect,
                                hipStream_t st) {
-  hipLaunchKernelGGL((bf_channels_kernel<NE, IQ>), dim3(tile_blocks(A)), dim3(256), 0, st, A,
-                     data, Y, mask, ev_first, n_slots, select, first, rect);
+  if (A.rx_window == 1)
+    hipLaunchKernelGGL((bf_channels_kernel<NE, IQ, true>), dim3(tile_blocks(A)), dim3(256), 0, st, A,
+                       data, Y, mask, ev_first, n_slots, select, first, rect);
+  else
+    hipLaunchKernelGGL((bf_channels_kernel<NE, IQ, false>), dim3(tile_blocks(A)), dim3(256), 0, st, A,
+                       data, Y, mask, ev_first, n_slots, select, first, rect);
 }
 
 template <bool IQ>

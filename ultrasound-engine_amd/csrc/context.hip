@@ -494,7 +494,7 @@ void enqueue_beamform(usx_ctx* c, const usx_bf_desc* b, const BfArgs& A, const S
     if (comb == 3) {
       timed(c, "apod_infer", [&] {
         usxd::launch_apod_infer(apod_net(c), n_pix, b->n_channels, Y, mk, d_frame, acc, cf,
-                                acc != nullptr, st);
+                                acc != nullptr, c->n_cu, st);
       });
       return;
     }

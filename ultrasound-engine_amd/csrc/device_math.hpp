@@ -125,9 +125,14 @@ __device__ __attribute__((noinline)) float window_value_slow(int w, float u, flo
 // the value is the reference's within a few 1e-8. Hann's 0.5 (1 - cos x) is
 // 0 exactly where the C library's cosf(x) returns 1.0f, i.e. for
 // x <= cos1_lo or x >= cos1_hi; rect and Hamming are non-zero on the whole
-// aperture; Tukey and Blackman take window_value.
+// aperture; Tukey and Blackman take window_value. HANN: the window is known
+// to be Hann (w == 1), so no call to the out-of-line window_value_slow is
+// compiled in -- a call in the channel loop makes the compiler keep the
+// loop's scalar values in VGPR lanes (a readlane per use).
+template <bool HANN = false>
 __device__ __forceinline__ float rx_weight(int w, float tukey_alpha, float dx, float d, float r,
                                            float cos1_lo, float cos1_hi, bool* on) {
+  if (HANN) w = 1;
   const float u = div_r(dx, d, r);
   if (u < -1.0f || u > 1.0f) {
     *on = false;
@@ -148,9 +153,13 @@ __device__ __forceinline__ float rx_weight(int w, float tukey_alpha, float dx, f
     *on = !(x <= cos1_lo || x >= cos1_hi);
     return 0.5f * (1.0f - cs);
   }
-  const float a = window_value_slow(w, u, tukey_alpha);
-  *on = !(a <= 0.0f);
-  return a;
+  if constexpr (HANN) {
+    return 0.0f;  // unreachable: w == 1
+  } else {
+    const float a = window_value_slow(w, u, tukey_alpha);
+    *on = !(a <= 0.0f);
+    return a;
+  }
 }
 
 // Correctly rounded square root of a positive normal float, by the

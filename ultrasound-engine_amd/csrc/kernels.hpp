@@ -138,9 +138,10 @@ struct ApodAdam {
 size_t apod_infer_smem();
 int apod_splits(int batch);
 void launch_apod_pack(const ApodNet& N, hipStream_t st);
+// n_cu: compute units of the device (the kernel is persistent, one workgroup per CU).
 void launch_apod_infer(const ApodNet& N, int n_pix, int n_ch, const float2* Y,
                        const uint32_t* mask, float2* frame, float* acc, int cf, int incoherent,
-                       hipStream_t st);
+                       int n_cu, hipStream_t st);
 void launch_apod_gather(int n_list, const int* pixels, int n_ch, const float2* Y,
                         const uint32_t* mask, const float2* weights, const int* status,
                         const ApodData& D, long dst, hipStream_t st);
