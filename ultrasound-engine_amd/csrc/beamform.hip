@@ -207,13 +207,13 @@ __device__ __forceinline__ RxAperture rx_aperture(const BfArgs& A, const Pt& p, 
 }
 // Receive weight of channel ch (*on: the reference's weight is non-zero).
 template <bool HANN>
-__device__ __forceinline__ float rx_apod(const BfArgs& A, const RxAperture& ap, int ch, const Pt& p,
+__device__ __forceinline__ float rx_apod(const BfArgs& A, const RxAperture& ap, float ex, const Pt& p,
                                         bool* on) {
   if (!ap.any) {
     *on = false;
     return 0.0f;
   }
-  return rx_weight<HANN>(A.rx_window, A.tukey_alpha, A.ex[ch] - p.x, ap.d, ap.r, A.cos1_lo, A.cos1_hi, on);
+  return rx_weight<HANN>(A.rx_window, A.tukey_alpha, ex - p.x, ap.d, ap.r, A.cos1_lo, A.cos1_hi, on);
 }
 
 // Conservative channel range of the receive aperture of p: |x_e - x_p| <= z/(2 F#).
@@ -239,6 +239,32 @@ __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, boo
   *lo = __builtin_amdgcn_readfirstlane(wave_min(l));
   *hi = __builtin_amdgcn_readfirstlane(wave_max(h));
 }
+
+// Element x of the channels a wavefront visits, ch = first + 4 i (i < 128,
+// i.e. up to 512 channels), one per lane in two registers: entry i (i
+// wavefront-uniform) is a readlane, with no memory access and no wait in
+// the channel loop. (For the channel-vector kernel, whose stores of Y keep
+// the compiler from scalarising a load of A.ex: a vector load and a
+// vmcnt(0) wait per channel.)
+struct WaveEx {
+  float r0, r1;
+  __device__ __forceinline__ void load(const BfArgs& A, int first, int lane) {
+    const int c0 = first + 4 * lane, c1 = first + 4 * (lane + 64);
+    r0 = c0 < A.n_ch ? A.ex[c0] : 0.0f;
+    r1 = c1 < A.n_ch ? A.ex[c1] : 0.0f;
+    // wait here (vmcnt(0)), once, so that the channel loop's readlanes of
+    // r0 / r1 carry no wait for these loads (which would also wait for the
+    // loop's own gathers and stores)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+  }
+  // (both readlanes, then a scalar select: a VALU select of r0 / r1 would be
+  // formed only in the active lanes, and lane i may be inactive here)
+  __device__ __forceinline__ float at(int i) const {
+    const int v0 = __builtin_amdgcn_readlane(__float_as_int(r0), i & 63);
+    const int v1 = __builtin_amdgcn_readlane(__float_as_int(r1), i & 63);
+    return __int_as_float(i < 64 ? v0 : v1);
+  }
+};
 
 }  // namespace
 
@@ -306,11 +332,11 @@ __device__ __forceinline__ bool gathers_inside(const BfArgs& A, const Pt& p, boo
 // sample index is per lane.
 template <int NE, bool IQ, bool SEL, bool CHK = true, bool T0Z = false>
 __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* data, const Pt& p,
-                                                const TxSlot (&sl)[NE], int nsl, int ch, float a,
-                                                int ev_first) {
+                                                const TxSlot (&sl)[NE], int nsl, int ch, float ex,
+                                                float a, int ev_first) {
   // (nsl <= NE slots are live; !CHK: gathers_inside holds for this wavefront;
   // T0Z: t0 == 0, where (tau + trx) - t0 is tau + trx exactly)
-  const float trx = CHK ? rx_time(A.ex[ch], p, A.c, A.inv_c) : rx_time_normal(A.ex[ch], p, A.c, A.inv_c);
+  const float trx = CHK ? rx_time(ex, p, A.c, A.inv_c) : rx_time_normal(ex, p, A.c, A.inv_c);
   const float nm1 = static_cast<float>(A.n_s - 1);
   float2 acc = make_float2(0.0f, 0.0f);
   // NE = 1 and NE = 11 are launched only with exactly that many slots
@@ -401,11 +427,13 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
     constexpr bool T0Z = decltype(t0_zero)::value;
   for (int ch = lo + w; ch <= hi; ch += 4) {
     bool on;
-    const float a = rx_apod<HANN>(A, ap, ch, p, &on);
+    // (a scalar load: this kernel stores nothing before its channel loop)
+    const float ex = A.ex[ch];
+    const float a = rx_apod<HANN>(A, ap, ex, p, &on);
     if (!on) continue;
     ++na;
-    const float2 y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                         : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, a, ev_first);
+    const float2 y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, ex, a, ev_first)
+                         : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, ex, a, ev_first);
     if (COMB == 0) {
       das.x += y.x;
       das.y += y.y;
@@ -519,6 +547,8 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   // wavefront is inside the traces, with them otherwise (gathers_inside is
   // wavefront-uniform and the four wavefronts share pixels and channel range,
   // so all take the same copy and meet the same barriers)
+  WaveEx exs{0.0f, 0.0f};
+  if (lo <= hi) exs.load(A, (lo & ~31) + w, lane);  // this wave's channels (lo & ~31) + w + 4 i
   auto channels = [&](auto checked, auto t0_zero) {
     constexpr bool CHK = decltype(checked)::value;
     constexpr bool T0Z = decltype(t0_zero)::value;
@@ -530,16 +560,17 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
         const int ch = c0 + cc;
         float2 y = make_float2(0.0f, 0.0f);
         if (valid && ch >= lo && ch <= hi) {
+          const float ex = exs.at((ch - (lo & ~31)) >> 2);
           bool on;
-          float a = rx_apod<HANN>(A, ap, ch, p, &on);
+          float a = rx_apod<HANN>(A, ap, ex, p, &on);
           if (rect) {  // beamformer.cpp:385: a > 0 ? 1 : 0, so a NaN weight is inactive here
             on = on && a == a;
             a = 1.0f;
           }
           if (on) {
             mybits |= 1u << cc;
-            y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, a, ev_first)
-                    : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, a, ev_first);
+            y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, ex, a, ev_first)
+                    : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, ex, a, ev_first);
           }
         }
         tile[lane][cc] = y;

@@ -230,10 +230,7 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->tile_slots < 0 || b->tile_slots > 16 || b->tile_mv_occupancy < 0 || b->tile_mv_occupancy > 32)
     fail(c, USX_ERR_INVALID, "beamform: tile_slots must be in 0..16 and tile_mv_occupancy in 0..32");
   if (b->combiner == 2) {
-    // the largest bucket (under the fast kernel's rule when it applies: L = 63
-    // and 64 then go to the general kernel's P = 80)
-    const bool fast = b->mv_precision == 0 && b->mv_range_average <= 2 && b->n_channels <= 256;
-    const int pmax = usxd::mv_max_bucket_p(b->n_channels, mv_sub(b), fast);
+    const int pmax = usxd::mv_max_bucket_p(b->n_channels, mv_sub(b));
     if (pmax > usxd::kMvMaxP)
       fail(c, USX_ERR_INVALID,
            "MV: subarray length up to %d with %d channels exceeds the supported %d",
@@ -435,7 +432,7 @@ ScratchSize scratch_size(const usx_bf_desc* b, const Tile& T) {
   if (incoh) z.acc = n_pix * sizeof(float);
   if (comb == 2) {
     z.counts = 16 * sizeof(int);
-    z.lists = static_cast<size_t>(usxd::mv_max_bucket_p(b->n_channels, mv_sub(b), true) / 16) * n_pix * sizeof(int);
+    z.lists = static_cast<size_t>(usxd::mv_max_bucket_p(b->n_channels, mv_sub(b)) / 16) * n_pix * sizeof(int);
   }
   return z;
 }

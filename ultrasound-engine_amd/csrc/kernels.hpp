@@ -69,26 +69,21 @@ struct MvArgs {
   int n_cu;        // compute units of the device (sizes the persistent grids)
   int n_xcd;       // XCDs of the device (the persistent grids give each a contiguous run of pixels)
 };
-// Whether the fast MV kernel (FP32, one matrix row per lane, the right-hand
-// sides solved as two rows bordering the matrix) serves the size buckets
-// P <= 64 of this configuration; the general kernel serves the rest.
+// Whether the fast MV kernel (FP32, one matrix row per lane) serves the size
+// buckets P <= 64 of this configuration; the general kernel serves the rest.
 __host__ __device__ inline bool mv_fast_config(const MvArgs& M) {
   return M.precision == 0 && M.weights == nullptr && M.range_avg >= 0 && M.range_avg <= 2 &&
          M.n_ch <= 256;
 }
-// Size bucket b (matrices padded to P = 16 (b + 1)) of a pixel whose subarray
-// length is L: the fast kernel's buckets hold L <= P - 2 (its two bordering
-// rows; L = 63, 64 go to the general kernel's P = 80), the general kernel's
-// L <= P.
-__host__ __device__ inline int mv_bucket(int L, bool fast) {
-  return (fast && L <= 64) ? (L + 17) / 16 - 1 : (L + 15) / 16 - 1;
-}
-// counts: 16 ints; lists: (mv_max_bucket_p(.., true) / 16) * n_pix ints (device scratch).
+// Pixel counts: counts[b] for size bucket b (matrices padded to
+// P = 16 (b + 1)), counts[kFullList + b] for the fast kernel's pixels of
+// L = P - 1 and P in bucket b (b < 4), which sit at the back of the bucket's
+// list. counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
+constexpr int kFullList = 8;
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
                int* counts, int* lists, hipStream_t st);
-// Largest covariance size bucket P a configuration can produce (fast: the
-// fast kernel's bucket rule, which never gives a smaller P).
-int mv_max_bucket_p(int n_ch, int subarray, bool fast);
+// Largest covariance size bucket (L rounded up to 16) a configuration can produce.
+int mv_max_bucket_p(int n_ch, int subarray);
 // Dynamic LDS per wavefront of the general MV kernel for bucket P (refine:
 // the full matrix, else its packed lower triangle).
 size_t mv_smem_bytes(int n_ch, int range_avg, int subarray, int P, bool refine);

@@ -64,6 +64,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -407,9 +408,13 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   }
   const bool take = valid && na >= 4;
   const int L = min(max(M.subarray, 2), max(2, na / 2));
-  const int b = take ? mv_bucket(L, mv_fast_config(M)) : -1;
-  const int slot = bucket_slot(counts, b, take);
-  if (take) lists[static_cast<size_t>(b) * n_pix + slot] = pix;
+  const int b = (L + 15) / 16 - 1;  // size bucket, P = 16 (b + 1)
+  // the fast kernel's pixels of L = P - 1, P: their own count, from the back
+  // of the bucket's array
+  const bool full = mv_fast_config(M) && b < 4 && L >= 16 * (b + 1) - 1;
+  const int cb = full ? kFullList + b : b;
+  const int slot = bucket_slot(counts, take ? cb : -1, take);
+  if (take) lists[static_cast<size_t>(b) * n_pix + (full ? n_pix - 1 - slot : slot)] = pix;
 }
 
 // Trailing update of the general path's Cholesky block [j0, j0 + 8) on MFMA,
@@ -1131,13 +1136,21 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   float2* A = rows + NR * RS;
   auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
 
-  const int count = counts[BUCKET];
+  // Two lists per bucket (mv_classify): the pixels of L <= P - 2 from the
+  // front of the bucket's array, solved with the right-hand sides as two
+  // rows bordering the matrix (BORDER), then those of L = P - 1 and P from
+  // its back, whose bordering rows would need lanes P and P + 1, with the
+  // forward substitutions carried through the column loop instead.
+  auto solve = [&](auto border_tag) {
+  constexpr bool BORDER = decltype(border_tag)::value;
+  const int count = counts[BORDER ? BUCKET : kFullList + BUCKET];
   const int* list = lists + static_cast<size_t>(BUCKET) * n_pix;
   for (int it = xcd_slot(M.n_xcd) * G; it < count; it += gridDim.x * G) {
     // a group past the end of the list repeats the wavefront's first pixel
     // and stores nothing
     const bool valid = it + g < count;
-    const int pix = list[valid ? it + g : it];
+    const int q = valid ? it + g : it;
+    const int pix = list[BORDER ? q : n_pix - 1 - q];
     __syncthreads();  // LDS of the previous pixels is free
     const Staged sg = stage_group<NR, RS, G>(M, Y, mask, pix, lane, act, rows);
     const int na = sg.na, nrow = sg.nrow;
@@ -1299,8 +1312,10 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       // L + 1 equal to conj(L^-1 1) and conj(L^-1 avg), so the column loop's
       // row-per-lane work does both forward substitutions (rows L + 1 < P:
       // the buckets hold L <= P - 2).
-      A[at(L, sub)] = make_float2(1.0f, 0.0f);
-      A[at(L + 1, sub)] = make_float2(b2r, -b2i);
+      if (BORDER) {
+        A[at(L, sub)] = make_float2(1.0f, 0.0f);
+        A[at(L + 1, sub)] = make_float2(b2r, -b2i);
+      }
     }
     const float avr = b2r, avi = b2i;  // the subarray average, kept for the refined weights
     __syncthreads();
@@ -1319,13 +1334,15 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     // pixels per wavefront) or on the VALU (two pixels per wavefront)
     constexpr bool kMfmaUpdate = G != 2;
     float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
+    // !BORDER: the forward substitutions' vectors, lane i holding entry i
+    float b1r = (!BORDER && sub < L) ? 1.0f : 0.0f, b1i = 0.0f;
     bool ok = true;  // this group's pivots so far are positive
-    const int L2 = L + 2;  // rows of the bordered matrix
+    const int L2 = BORDER ? L + 2 : L;  // rows (of the bordered matrix)
     for (int j0 = 0; j0 < Lw; j0 += CB) {
-      // Lanes outside [j0, L + 2) keep zero partial sums, so their factor
-      // entries are 0.
+      // Lanes outside [j0, L2) keep zero partial sums, so their factor
+      // entries are 0 (and !BORDER: their b stays as it is).
       if (kMfmaUpdate && j0 > 0) {
-        block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, Lw + 2, L);
+        block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, BORDER ? Lw + 2 : Lw, L);
         __syncthreads();
       }
       float sr[CB], si[CB], lr[CB], li[CB];
@@ -1387,9 +1404,23 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         if (REF) rjj = piv ? rj : rjj;
         // rows not below the pivot take l = 0: their entries of this column are
         // never read (rows above j are not stored, and the factor's diagonal is
-        // not read back)
-        lr[c] = below ? sr[c] * rj : 0.0f;
-        li[c] = below ? si[c] * rj : 0.0f;
+        // not read back), so b -= l y leaves them as they are without a select
+        const float ar = below ? sr[c] * rj : 0.0f, ai = below ? si[c] * rj : 0.0f;
+        lr[c] = ar;
+        li[c] = ai;
+        if constexpr (!BORDER) {
+          // y_j = b_j / L_jj at the pivot row, b_i -= L_ij y_j below it
+          const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
+          const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
+          const float n1r = fmaf(-ar, y1r, fmaf(ai, y1i, b1r));
+          const float n1i = fmaf(-ar, y1i, fmaf(-ai, y1r, b1i));
+          const float n2r = fmaf(-ar, y2r, fmaf(ai, y2i, b2r));
+          const float n2i = fmaf(-ar, y2i, fmaf(-ai, y2r, b2i));
+          b1r = piv ? y1r : n1r;
+          b1i = piv ? y1i : n1i;
+          b2r = piv ? y2r : n2r;
+          b2i = piv ? y2i : n2i;
+        }
       }
       if (sub >= j0 && sub < L2) {
 #pragma unroll
@@ -1401,10 +1432,16 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     }
     const bool on = sub < L;
     const float nf = static_cast<float>(na);
-    // rows L and L + 1 of the factor: e1 = conj(y1), e2 = conj(y2) with
-    // y1 = L^-1 1 and y2 = L^-1 avg
-    const float2 e1 = on ? A[at(L, sub)] : make_float2(0.0f, 0.0f);
-    const float2 e2 = on ? A[at(L + 1, sub)] : make_float2(0.0f, 0.0f);
+    // e1 = conj(y1), e2 = conj(y2) with y1 = L^-1 1 and y2 = L^-1 avg: rows L
+    // and L + 1 of the bordered factor, or the substitutions' vectors
+    float2 e1, e2;
+    if constexpr (BORDER) {
+      e1 = on ? A[at(L, sub)] : make_float2(0.0f, 0.0f);
+      e2 = on ? A[at(L + 1, sub)] : make_float2(0.0f, 0.0f);
+    } else {
+      e1 = on ? make_float2(b1r, -b1i) : make_float2(0.0f, 0.0f);
+      e2 = on ? make_float2(b2r, -b2i) : make_float2(0.0f, 0.0f);
+    }
     bool das;
     float2 out;
     if constexpr (!REF) {
@@ -1524,12 +1561,14 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       store_out(M, pix, out, frame, acc);
     }
   }
+  };
+  solve(std::true_type{});
+  solve(std::false_type{});
 }
 
 template <int P>
 static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame,
                           float* acc, const int* counts, const int* lists, hipStream_t st) {
-  // (mv_classify put pixels of L <= P - 2 here when this holds, else L <= P)
   const bool fast = P <= 64 && mv_fast_config(M);
   const size_t smem = fast ? mv_fast_smem_bytes(M.n_ch, M.range_avg, M.subarray, P)
                            : mv_smem_bytes(M.n_ch, M.range_avg, M.subarray, P, M.refine != 0);
@@ -1596,9 +1635,9 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
 #undef USX_LAUNCH
 }
 
-int mv_max_bucket_p(int n_ch, int subarray, bool fast) {
+int mv_max_bucket_p(int n_ch, int subarray) {
   const int lmax = std::min(std::max(subarray, 2), std::max(2, n_ch / 2));
-  return 16 * (mv_bucket(lmax, fast) + 1);
+  return (lmax + 15) / 16 * 16;
 }
 
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
@@ -1607,7 +1646,7 @@ void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* f
   hipLaunchKernelGGL(mv_zero_counts, dim3(1), dim3(64), 0, st, counts);
   hipLaunchKernelGGL(mv_classify, dim3((n_pix + 255) / 256), dim3(256), 0, st, M, Y, mask, frame,
                      acc, counts, lists);
-  const int pmax = mv_max_bucket_p(M.n_ch, M.subarray, mv_fast_config(M));
+  const int pmax = mv_max_bucket_p(M.n_ch, M.subarray);
   if (pmax >= 16) launch_bucket<16>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 32) launch_bucket<32>(M, Y, mask, frame, acc, counts, lists, st);
   if (pmax >= 48) launch_bucket<48>(M, Y, mask, frame, acc, counts, lists, st);
