@@ -1320,19 +1320,15 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     const float avr = b2r, avi = b2i;  // the subarray average, kept for the refined weights
     __syncthreads();
 
-    // ---- Cholesky of the bordered matrix, left-looking in blocks of CB
-    // columns over the L columns of R and the L + 2 rows. For the block
-    // [j0, j0 + CB) the terms of the finished columns,
+    // ---- Cholesky (of the bordered matrix when BORDER), left-looking in
+    // blocks of CB columns over the L columns of R and its L (L + 2) rows. For
+    // the block [j0, j0 + CB) the terms of the finished columns,
     // s_c = A(i, j0+c) - sum_{k<j0} L(i,k) conj(L(j0+c,k)), are formed on MFMA
-    // in the triangle (block_update_mfma), or by lane i for its own row (one
-    // read of its row per k feeds CB accumulators, k increasing); lane i then
-    // finishes its row of the block column by column with the in-block terms
-    // k = j0 .. j0+c-1 (taken by readlane from the registers of lane j0+c), in
-    // increasing k as the reference (beamformer.cpp:48-63).
+    // in the triangle (block_update_mfma); lane i then finishes its row of the
+    // block column by column with the in-block terms k = j0 .. j0+c-1 (taken
+    // by readlane from the registers of lane j0+c), in increasing k as the
+    // reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
-    // the finished columns' terms on MFMA (measured faster for one and four
-    // pixels per wavefront) or on the VALU (two pixels per wavefront)
-    constexpr bool kMfmaUpdate = G != 2;
     float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
     // !BORDER: the forward substitutions' vectors, lane i holding entry i
     float b1r = (!BORDER && sub < L) ? 1.0f : 0.0f, b1i = 0.0f;
@@ -1341,7 +1337,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     for (int j0 = 0; j0 < Lw; j0 += CB) {
       // Lanes outside [j0, L2) keep zero partial sums, so their factor
       // entries are 0 (and !BORDER: their b stays as it is).
-      if (kMfmaUpdate && j0 > 0) {
+      if (j0 > 0) {
         block_update_mfma<P, G>(smem_raw, group_bytes, tri_off, lane, j0, BORDER ? Lw + 2 : Lw, L);
         __syncthreads();
       }
@@ -1361,23 +1357,6 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           const float2 v = A[on ? at(sub, j0 + c) : 0];
           sr[c] = on ? v.x : 0.0f;
           si[c] = on ? v.y : 0.0f;
-        }
-      }
-      if (sub >= j0 && sub < L2) {
-        if constexpr (!kMfmaUpdate) {
-#pragma unroll 2
-          for (int k = 0; k < j0; ++k) {
-            // column k: this lane's row, and the block's rows j0 .. j0+CB-1 (contiguous;
-            // rows >= L hold stale values whose sums are never used, and j0 + CB <= P)
-            const float2* colk = A + at(0, k);
-            const float2 a = colk[sub];
-#pragma unroll
-            for (int c = 0; c < CB; ++c) {
-              const float2 b = colk[j0 + c];  // -= a conj(b)
-              sr[c] = fmaf(-a.x, b.x, fmaf(-a.y, b.y, sr[c]));
-              si[c] = fmaf(-a.y, b.x, fmaf(a.x, b.y, si[c]));
-            }
-          }
         }
       }
       // Column by column, branch-free (selects): rows below the pivot take
