@@ -1362,9 +1362,12 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       // Column by column, branch-free (selects): rows below the pivot take
       // L_ij (rows L and L + 1 included: the forward substitutions). A group
       // whose pixel has no column j (j >= L) changes nothing.
+      // (the block's first column as a scalar, so every readlane's lane
+      // select is an SGPR sum, with no per-column readfirstlane and hazard wait)
+      const int j0s = __builtin_amdgcn_readfirstlane(j0);
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
-        const int j = j0 + c;
+        const int j = j0s + c;
         if (j >= Lw) break;
         const bool col = j < L;
 #pragma unroll
