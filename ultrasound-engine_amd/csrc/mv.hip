@@ -205,6 +205,18 @@ __device__ __forceinline__ float group_bcast(float v, int k, int lane) {
   }
   return r;
 }
+// group_bcast for values produced well before their use: with several
+// groups one ds_bpermute (the LDS crossbar, no memory access) instead of G
+// readlanes and G - 1 selects; one readlane for a single group.
+template <int G>
+__device__ __forceinline__ float group_bcast_lds(float v, int k, int lane) {
+  if constexpr (G == 1) {
+    return lane_bcast(v, k);
+  } else {
+    constexpr int LP = 64 / G;
+    return __shfl(v, (lane & ~(LP - 1)) + k, 64);
+  }
+}
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
@@ -1372,7 +1384,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         const bool col = j < L;
 #pragma unroll
         for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
-          const float br = group_bcast<G>(lr[q], j, lane), bi = group_bcast<G>(li[q], j, lane);
+          const float br = group_bcast_lds<G>(lr[q], j, lane), bi = group_bcast_lds<G>(li[q], j, lane);
           sr[c] = fmaf(-lr[q], br, fmaf(-li[q], bi, sr[c]));
           si[c] = fmaf(-li[q], br, fmaf(lr[q], bi, si[c]));
         }
