@@ -192,24 +192,13 @@ __device__ __forceinline__ Staged stage_pixel(const MvArgs& M, const float2* __r
 }
 
 // Lanes of a wavefront split into G groups of LP = 64 / G lanes, one pixel
-// per group. group_bcast: v of lane k of the caller's group (readlanes and
-// selects); group_sum, group_sum_d: the sum over the caller's group.
+// per group. group_bcast: v of lane k of the caller's group -- one readlane
+// for a single group; with several groups one ds_bpermute (the LDS crossbar,
+// no memory access; measured faster than G readlanes and G - 1 selects even
+// on the Cholesky's critical path); group_sum, group_sum_d: the sum over the
+// caller's group.
 template <int G>
 __device__ __forceinline__ float group_bcast(float v, int k, int lane) {
-  constexpr int LP = 64 / G;
-  float r = lane_bcast(v, k);
-#pragma unroll
-  for (int g = 1; g < G; ++g) {
-    const float x = lane_bcast(v, g * LP + k);
-    r = lane >= g * LP ? x : r;
-  }
-  return r;
-}
-// group_bcast for values produced well before their use: with several
-// groups one ds_bpermute (the LDS crossbar, no memory access) instead of G
-// readlanes and G - 1 selects; one readlane for a single group.
-template <int G>
-__device__ __forceinline__ float group_bcast_lds(float v, int k, int lane) {
   if constexpr (G == 1) {
     return lane_bcast(v, k);
   } else {
@@ -247,8 +236,8 @@ __device__ __forceinline__ int xcd_slot(int X) {
 template <int G>
 __device__ __forceinline__ double group_bcast_d(double v, int k, int lane) {
   const long long u = __double_as_longlong(v);
-  const float lo = group_bcast_lds<G>(__int_as_float(static_cast<int>(u)), k, lane);
-  const float hi = group_bcast_lds<G>(__int_as_float(static_cast<int>(u >> 32)), k, lane);
+  const float lo = group_bcast<G>(__int_as_float(static_cast<int>(u)), k, lane);
+  const float hi = group_bcast<G>(__int_as_float(static_cast<int>(u >> 32)), k, lane);
   return __longlong_as_double((static_cast<long long>(__float_as_int(hi)) << 32) |
                               static_cast<unsigned int>(__float_as_int(lo)));
 }
@@ -1384,7 +1373,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         const bool col = j < L;
 #pragma unroll
         for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
-          const float br = group_bcast_lds<G>(lr[q], j, lane), bi = group_bcast_lds<G>(li[q], j, lane);
+          const float br = group_bcast<G>(lr[q], j, lane), bi = group_bcast<G>(li[q], j, lane);
           sr[c] = fmaf(-lr[q], br, fmaf(-li[q], bi, sr[c]));
           si[c] = fmaf(-li[q], br, fmaf(lr[q], bi, si[c]));
         }
@@ -1404,8 +1393,8 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         li[c] = ai;
         if constexpr (!BORDER) {
           // y_j = b_j / L_jj at the pivot row, b_i -= L_ij y_j below it
-          const float y1r = group_bcast_lds<G>(b1r, j, lane) * rj, y1i = group_bcast_lds<G>(b1i, j, lane) * rj;
-          const float y2r = group_bcast_lds<G>(b2r, j, lane) * rj, y2i = group_bcast_lds<G>(b2i, j, lane) * rj;
+          const float y1r = group_bcast<G>(b1r, j, lane) * rj, y1i = group_bcast<G>(b1i, j, lane) * rj;
+          const float y2r = group_bcast<G>(b2r, j, lane) * rj, y2i = group_bcast<G>(b2i, j, lane) * rj;
           const float n1r = fmaf(-ar, y1r, fmaf(ai, y1i, b1r));
           const float n1i = fmaf(-ar, y1i, fmaf(-ai, y1r, b1i));
           const float n2r = fmaf(-ar, y2r, fmaf(ai, y2i, b2r));
@@ -1456,7 +1445,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       auto forward = [&](float& zr, float& zi) {  // z <- L^-1 z
         for (int j = 0; j < Lw; ++j) {
           const bool col = j < L;
-          const float yr = group_bcast_lds<G>(zr * rjj, j, lane), yi = group_bcast_lds<G>(zi * rjj, j, lane);
+          const float yr = group_bcast<G>(zr * rjj, j, lane), yi = group_bcast<G>(zi * rjj, j, lane);
           const bool below = col && sub > j && sub < L;
           const float2 l = A[below ? at(sub, j) : 0];  // L(i, j)
           const float nr_ = fmaf(-l.x, yr, fmaf(l.y, yi, zr)), ni_ = fmaf(-l.x, yi, fmaf(-l.y, yr, zi));
@@ -1472,7 +1461,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       auto backward = [&](float& zr, float& zi) {  // z <- L^-H z
         for (int j = Lw - 1; j >= 0; --j) {
           const bool col = j < L;
-          const float xr = group_bcast_lds<G>(zr * rjj, j, lane), xi = group_bcast_lds<G>(zi * rjj, j, lane);
+          const float xr = group_bcast<G>(zr * rjj, j, lane), xi = group_bcast<G>(zi * rjj, j, lane);
           const bool above = col && sub < j;
           const float2 l = A[above ? at(j, sub) : 0];  // L(j, i): z_i -= conj(L(j, i)) x_j
           const float nr_ = fmaf(-l.x, xr, fmaf(-l.y, xi, zr)), ni_ = fmaf(-l.x, xi, fmaf(l.y, xr, zi));
@@ -1494,7 +1483,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           const int s = s0 + sub;
           double tr = 0.0, ti = 0.0;  // t_s = x_s^H w
           for (int k = 0; k < Lw; ++k) {
-            const float wkr = group_bcast_lds<G>(wr, k, lane), wki = group_bcast_lds<G>(wi, k, lane);
+            const float wkr = group_bcast<G>(wr, k, lane), wki = group_bcast<G>(wi, k, lane);
             if (k < L && s < ns) {
               const float2 v = x[s + k];
               tr += static_cast<double>(v.x) * wkr + static_cast<double>(v.y) * wki;
