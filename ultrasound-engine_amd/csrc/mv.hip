@@ -1348,14 +1348,20 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         sr[c] = 0.0f;
         si[c] = 0.0f;
       }
+      // (the block's first column as a scalar, so every column base and every
+      // readlane's lane select is scalar arithmetic, with no per-column
+      // readfirstlane and hazard wait and no per-lane triangular indexing)
+      const int j0s = __builtin_amdgcn_readfirstlane(j0);
       {
-        // branch-free: every lane reads (its own entry, or entry 0) and keeps
+        // branch-free: every lane reads (its own entry, or the triangle's
+        // last: a scalar column base plus the lane's row, clamped) and keeps
         // the value where its row reaches the column
-        const bool rowon = sub >= j0 && sub < L2;
+        const bool rowon = sub >= j0s && sub < L2;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-          const bool on = rowon && j0 + c <= sub;
-          const float2 v = A[on ? at(sub, j0 + c) : 0];
+          const int jc = j0s + c;
+          const bool on = rowon && jc <= sub;
+          const float2 v = A[min(at(0, jc) + sub, TRI - 1)];
           sr[c] = on ? v.x : 0.0f;
           si[c] = on ? v.y : 0.0f;
         }
@@ -1363,9 +1369,6 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       // Column by column, branch-free (selects): rows below the pivot take
       // L_ij (rows L and L + 1 included: the forward substitutions). A group
       // whose pixel has no column j (j >= L) changes nothing.
-      // (the block's first column as a scalar, so every readlane's lane
-      // select is an SGPR sum, with no per-column readfirstlane and hazard wait)
-      const int j0s = __builtin_amdgcn_readfirstlane(j0);
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         const int j = j0s + c;
@@ -1405,10 +1408,10 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           b2i = piv ? y2i : n2i;
         }
       }
-      if (sub >= j0 && sub < L2) {
+      if (sub >= j0s && sub < L2) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
-          if (j0 + c < L && j0 + c <= sub) A[at(sub, j0 + c)] = make_float2(lr[c], li[c]);
+          if (j0s + c < L && j0s + c <= sub) A[at(0, j0s + c) + sub] = make_float2(lr[c], li[c]);
       }
       __syncthreads();
       if (!__any(ok)) break;  // every group's factorisation failed
