@@ -995,6 +995,12 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
 // * No back substitution: with w = R^-1 1, sum w = 1^H R^-1 1 = |y1|^2 and
 //   w^H avg = y1^H y2 (y1 = L^-1 1, y2 = L^-1 avg), so
 //   out = (y1^H y2) / |y1|^2 * n.
+// The fast path's lower triangle, packed by rows: (i >= k, k) at
+// i (i + 1) / 2 + k. A row's entries are contiguous, so one step along the
+// columns is one element for every row, and the operands of the MFMA trailing
+// update sit at fixed distances from a per-lane base (immediate offsets).
+__device__ __forceinline__ int tri_at(int i, int k) { return ((i * (i + 1)) >> 1) + k; }
+
 // Trailing update of the Cholesky block [j0, j0 + 8) on MFMA, for every
 // pixel group gg of the wavefront: S(r, j0 + c) = A(r, j0 + c) -
 // sum_{k < j0} L(r, k) conj(L(j0 + c, k)), written back into the packed lower
@@ -1005,14 +1011,13 @@ __global__ __launch_bounds__(64) void mv_solve(MvArgs M, const float2* __restric
 // rows per tile, 4 real k (2 complex columns) per step, accumulated onto the
 // triangle's entries with B negated. Lane l supplies A[row l % 16][k-part
 // l / 16] and B[k-part l / 16][column l % 16] and holds D[4 (l / 16) + i][l % 16].
-// Tiles and steps have wavefront-uniform bounds; rows and columns a group
-// does not own produce values that are never stored. tri_off: byte offset of
-// the triangle in a group's LDS, group_bytes: LDS per group.
-template <int P, int G>
-__device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t group_bytes,
-                                                  size_t tri_off, int lane, int j0, int Lw,
-                                                  int L) {
-  constexpr int LP = 64 / G;
+// The row tiles from the block's down to the last are accumulated together
+// (NT of them), so a step's B operand is read and signed once for all; with
+// the triangle packed by rows every operand of step st lies 4 st floats past
+// its lane's base. Tiles and steps have wavefront-uniform bounds; rows and
+// columns a group does not own produce values that are never stored.
+template <int NT>
+__device__ __forceinline__ void block_update_tiles(float* tri, int lane, int j0, int t0, int Lg) {
   const int lr = lane & 15, lq = lane >> 4;
   const int c = lr >> 1, part = lr & 1;
   const int kp = lq & 1, kh = lq >> 1;  // this lane's complex column k = 2 st + kh
@@ -1020,36 +1025,55 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
   // (1,0), (1,1): one component of b times a lane constant sign
   const int comp = part ^ kp;
   const float sgn = (part == 1 && kp == 0) ? 1.0f : -1.0f;
-  const int t0 = j0 >> 4, t1 = (Lw + 15) >> 4;
-  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
+  const int col = j0 + c;
+  int ib = 2 * tri_at(col, kh) + comp;  // float indices into tri (kept LDS-typed)
+  int ia[NT];
+  f4 acc[NT];
+#pragma unroll
+  for (int m = 0; m < NT; ++m) {
+    ia[m] = 2 * tri_at(16 * (t0 + m) + lr, kh) + kp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[m][i] = tri[2 * tri_at(16 * (t0 + m) + 4 * lq + i, col) + part];
+  }
+  for (int s4 = 0; s4 < (j0 >> 1); s4 += 4) {  // j0 / 2 is a multiple of 4
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float b = sgn * tri[ib + 4 * u];
+#pragma unroll
+      for (int m = 0; m < NT; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(tri[ia[m] + 4 * u], b, acc[m], 0, 0, 0);
+    }
+    ib += 16;
+#pragma unroll
+    for (int m = 0; m < NT; ++m) ia[m] += 16;
+  }
+#pragma unroll
+  for (int m = 0; m < NT; ++m) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * (t0 + m) + 4 * lq + i;
+      if (r >= col && col < Lg) tri[2 * tri_at(r, col) + part] = acc[m][i];
+    }
+  }
+}
+
+// tri_off: byte offset of the triangle in a group's LDS, group_bytes: LDS per
+// group; Lw: the wavefront's largest row count.
+template <int P, int G>
+__device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t group_bytes,
+                                                  size_t tri_off, int lane, int j0, int Lw,
+                                                  int L) {
+  constexpr int LP = 64 / G;
+  constexpr int T = P / 16;
+  const int t0 = j0 >> 4, nt = min((Lw + 15) >> 4, T) - t0;  // >= 1: j0 < Lw
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     float* tri = reinterpret_cast<float*>(smem + gg * group_bytes + tri_off);
     const int Lg = G == 1 ? L : __shfl(L, gg * LP, 64);
-    for (int mt = t0; mt < t1; ++mt) {
-      f4 acc;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = tri[2 * at(16 * mt + 4 * lq + i, j0 + c) + part];
-      // float index of L(row, k) part kp, advanced incrementally: at(i, k + 2) -
-      // at(i, k) = 2P - 2k - 3, which falls by 4 per step; L(j0 + c, k) part comp
-      // sits a fixed distance db away (at(i, k) - at(i', k) = i - i')
-      const int row = 16 * mt + lr;
-      int ia = 2 * at(row, kh) + kp;
-      int step = 2 * (2 * P - 2 * kh - 3);
-      const int db = 2 * (j0 + c - row) + comp - kp;
-#pragma unroll 4  // the steps' LDS reads in flight together (j0 / 2 is a multiple of 4)
-      for (int st = 0; st < (j0 >> 1); ++st) {
-        const float a = tri[ia], b = tri[ia + db];
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sgn * b, acc, 0, 0, 0);
-        ia += step;
-        step -= 8;
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 16 * mt + 4 * lq + i;
-        if (r >= j0 + c && j0 + c < Lg) tri[2 * at(r, j0 + c) + part] = acc[i];
-      }
-    }
+    if (T >= 4 && nt >= 4) block_update_tiles<T >= 4 ? 4 : 1>(tri, lane, j0, t0, Lg);
+    else if (T >= 3 && nt == 3) block_update_tiles<T >= 3 ? 3 : 1>(tri, lane, j0, t0, Lg);
+    else if (T >= 2 && nt == 2) block_update_tiles<T >= 2 ? 2 : 1>(tri, lane, j0, t0, Lg);
+    else block_update_tiles<1>(tri, lane, j0, t0, Lg);
   }
 }
 
@@ -1077,7 +1101,6 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
   // negated: component part ^ comp of w times a lane constant sign
   const int bcomp = part ^ comp;
   const float bsgn = ((part == 1 && comp == 0) ? -1.0f : 1.0f) * (hi ? 1.0f : -1.0f);
-  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const float* rows = reinterpret_cast<const float*>(smem + gg * group_bytes + rows_off);
@@ -1103,7 +1126,7 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = 16 * mt + 4 * lq + e;
-          if (row >= col && row < Lg) tri[2 * at(row, col) + part] = acc[e];
+          if (row >= col && row < Lg) tri[2 * tri_at(row, col) + part] = acc[e];
         }
       }
     }
@@ -1126,8 +1149,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   const int n_ch = M.n_ch;
   const int n_pix = M.n0 * M.n1;
   // per group: act (n_ch ints), NR rows at stride RS, the lower triangle
-  // packed by columns -- (i >= k, k) at k P - k (k + 1) / 2 + i, so a
-  // column's rows are contiguous (lane-consecutive reads and stores)
+  // packed by rows (tri_at)
   const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
   const size_t group_bytes = ints + sizeof(float2) * (NR * RS + TRI);
   const size_t tri_off = ints + sizeof(float2) * (NR * RS);
@@ -1135,7 +1157,8 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   int* act = reinterpret_cast<int*>(mine);
   float2* rows = reinterpret_cast<float2*>(mine + ints);
   float2* A = rows + NR * RS;
-  auto at = [](int i, int k) { return k * P - ((k * (k + 1)) >> 1) + i; };
+  auto at = [](int i, int k) { return tri_at(i, k); };
+  const int rb = tri_at(sub, 0);  // this lane's row
 
   // Two lists per bucket (mv_classify): the pixels of L <= P - 2 from the
   // front of the bucket's array, solved with the right-hand sides as two
@@ -1260,7 +1283,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
     // own diagonal, so the reads of 8 steps go out before their stores.
     walk_increments_mfma<P, NR, RS, G>(smem_raw, group_bytes, ints, tri_off, lane, L, ns);
     __syncthreads();
-    int pos = sub;  // at(sub + t, t); at(i + 1, k + 1) = at(i, k) + P - k
+    int pos = rb;  // at(sub + t, t); at(i + 1, k + 1) = at(i, k) + i + 2
     for (int t0 = 0; t0 < Lw; t0 += 8) {
       float2 d[8];
       int pu[8];
@@ -1268,7 +1291,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       for (int u = 0; u < 8; ++u) {
         const int t = t0 + u;
         pu[u] = pos;
-        pos += P - t;
+        pos += sub + t + 2;
         d[u] = A[min(pu[u], TRI - 1)];  // past the diagonal's end: read, never stored
       }
 #pragma unroll
@@ -1348,20 +1371,20 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
         sr[c] = 0.0f;
         si[c] = 0.0f;
       }
-      // (the block's first column as a scalar, so every column base and every
-      // readlane's lane select is scalar arithmetic, with no per-column
-      // readfirstlane and hazard wait and no per-lane triangular indexing)
+      // (the block's first column as a scalar, so every column offset and
+      // every readlane's lane select is scalar arithmetic, with no per-column
+      // readfirstlane and hazard wait)
       const int j0s = __builtin_amdgcn_readfirstlane(j0);
       {
         // branch-free: every lane reads (its own entry, or the triangle's
-        // last: a scalar column base plus the lane's row, clamped) and keeps
-        // the value where its row reaches the column
+        // last: its row's base plus a scalar column, clamped) and keeps the
+        // value where its row reaches the column
         const bool rowon = sub >= j0s && sub < L2;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
           const int jc = j0s + c;
           const bool on = rowon && jc <= sub;
-          const float2 v = A[min(at(0, jc) + sub, TRI - 1)];
+          const float2 v = A[min(rb + jc, TRI - 1)];
           sr[c] = on ? v.x : 0.0f;
           si[c] = on ? v.y : 0.0f;
         }
@@ -1411,7 +1434,7 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
       if (sub >= j0s && sub < L2) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
-          if (j0s + c < L && j0s + c <= sub) A[at(0, j0s + c) + sub] = make_float2(lr[c], li[c]);
+          if (j0s + c < L && j0s + c <= sub) A[rb + j0s + c] = make_float2(lr[c], li[c]);
       }
       __syncthreads();
       if (!__any(ok)) break;  // every group's factorisation failed
