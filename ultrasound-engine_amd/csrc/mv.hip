@@ -334,13 +334,22 @@ static int mv_fast_row_stride(int n_ch, int subarray, int P) {
 // 64 lanes, one matrix row per lane.
 static int mv_fast_groups(int P) { return P <= 16 ? 4 : (P <= 32 ? 2 : 1); }
 
-// Bytes of dynamic LDS for mv_solve_fast: per pixel group, the active list,
-// NR rows at the fixed stride and the packed lower triangle.
+// Bytes of mv_solve_fast's active list per pixel group: n_ch ints, and at
+// least lp float2 (one sink entry per lane for the covariance walks, which
+// run after the list is spent), a multiple of 16.
+__host__ __device__ inline size_t mv_fast_ints_bytes(int n_ch, int lp) {
+  const size_t a = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16, b = static_cast<size_t>(lp) * 8;
+  return a > b ? a : b;
+}
+
+// Bytes of dynamic LDS for mv_solve_fast: per pixel group, the packed lower
+// triangle, the active list and NR rows at the fixed stride.
 size_t mv_fast_smem_bytes(int n_ch, int range_avg, int subarray, int P) {
-  const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  return static_cast<size_t>(mv_fast_groups(P)) *
-         (ints + sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * mv_fast_row_stride(n_ch, subarray, P) +
-                                   static_cast<size_t>(P) * (P + 1) / 2));
+  const int groups = mv_fast_groups(P);
+  return static_cast<size_t>(groups) *
+         (mv_fast_ints_bytes(n_ch, 64 / groups) +
+          sizeof(float2) * (static_cast<size_t>(2 * range_avg + 1) * mv_fast_row_stride(n_ch, subarray, P) +
+                            static_cast<size_t>(P) * (P + 1) / 2));
 }
 
 __global__ void mv_zero_counts(int* counts) {
@@ -1087,12 +1096,13 @@ __device__ __forceinline__ void block_update_mfma(unsigned char* smem, size_t gr
 // columns per tile, and per MFMA the 4 real terms of one row r (high re, high
 // im, low re, low im -- the low ones with B negated). Indices are clamped to
 // L - 2, so every read lies in the staged entries 0 .. n - 1; row L - 1 gets
-// values the walks overwrite unread. rows_off / tri_off: byte offsets of the
+// values the walks overwrite unread. The increments come out multiplied by
+// the group's scale (B carries it). rows_off / tri_off: byte offsets of the
 // rows and of the triangle in a group's LDS, group_bytes: LDS per group.
 template <int P, int NR, int RS, int G>
 __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t group_bytes,
                                                      size_t rows_off, size_t tri_off, int lane,
-                                                     int L, int ns) {
+                                                     int L, int ns, float scale) {
   constexpr int LP = 64 / G;
   const int lr = lane & 15, lq = lane >> 4;
   const int c = lr >> 1, part = lr & 1;
@@ -1107,6 +1117,7 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
     float* tri = reinterpret_cast<float*>(smem + gg * group_bytes + tri_off);
     const int Lg = G == 1 ? L : __shfl(L, gg * LP, 64);
     const int nsg = G == 1 ? ns : __shfl(ns, gg * LP, 64);  // every lane takes part
+    const float bs = bsgn * (G == 1 ? scale : __shfl(scale, gg * LP, 64));
     const int off = hi ? nsg : 0;
     const int t1 = (Lg + 15) >> 4, c1 = (Lg + 7) >> 3;
     for (int mt = 0; mt < t1; ++mt) {
@@ -1120,7 +1131,7 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
         f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bsgn * rows[2 * (r * RS + j) + bcomp],
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bs * rows[2 * (r * RS + j) + bcomp],
                                                      acc, 0, 0, 0);
         const int col = 8 * cb + c;
 #pragma unroll
@@ -1148,15 +1159,16 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   const int g = lane / LP, sub = lane % LP;
   const int n_ch = M.n_ch;
   const int n_pix = M.n0 * M.n1;
-  // per group: act (n_ch ints), NR rows at stride RS, the lower triangle
-  // packed by rows (tri_at)
-  const size_t ints = (static_cast<size_t>(n_ch) * 4 + 15) / 16 * 16;
-  const size_t group_bytes = ints + sizeof(float2) * (NR * RS + TRI);
-  const size_t tri_off = ints + sizeof(float2) * (NR * RS);
+  // per group: the lower triangle packed by rows (tri_at), act (n_ch ints,
+  // at least LP float2: once staged, the walks' sinks A[TRI + m]), NR rows at
+  // stride RS
+  const size_t ints = mv_fast_ints_bytes(n_ch, LP);
+  const size_t group_bytes = sizeof(float2) * (TRI + NR * RS) + ints;
+  const size_t tri_off = 0, rows_off = sizeof(float2) * TRI + ints;
   unsigned char* mine = smem_raw + g * group_bytes;
-  int* act = reinterpret_cast<int*>(mine);
-  float2* rows = reinterpret_cast<float2*>(mine + ints);
-  float2* A = rows + NR * RS;
+  float2* A = reinterpret_cast<float2*>(mine);
+  int* act = reinterpret_cast<int*>(mine + sizeof(float2) * TRI);
+  float2* rows = reinterpret_cast<float2*>(mine + rows_off);
   auto at = [](int i, int k) { return tri_at(i, k); };
   const int rb = tri_at(sub, 0);  // this lane's row
 
@@ -1274,32 +1286,38 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
           }
         }
       }
-      cr = ar[0] + ar[1];
-      ci = ai[0] + ai[1];
+      cr = (ar[0] + ar[1]) * inv;
+      ci = (ai[0] + ai[1]) * inv;
     }
-    // Walk increments D on MFMA into the triangle, then lane m walks its
-    // sub-diagonal in place: at (m+t, t) it reads D(m+t, t), stores R(m+t, t)
-    // and adds, R(m+t+1, t+1) = R(m+t, t) + D(m+t, t). A lane touches only its
-    // own diagonal, so the reads of 8 steps go out before their stores.
-    walk_increments_mfma<P, NR, RS, G>(smem_raw, group_bytes, ints, tri_off, lane, L, ns);
+    // Walk increments D (times 1 / N) on MFMA into the triangle, then lane m
+    // walks its sub-diagonal in place: at (m+t, t) it reads D(m+t, t), stores
+    // R(m+t, t) and adds, R(m+t+1, t+1) = R(m+t, t) + D(m+t, t). A lane touches
+    // only its own diagonal, so the reads of 8 steps go out before their
+    // stores. Past its pixel's rows a lane walks on through rows the solve
+    // never reads (overwritten first where they border the matrix), and past
+    // the triangle it reads and writes its sink entry A[TRI + m] in the spent
+    // active list (or an entry between TRI and there, garbage for garbage):
+    // the stores need no condition, share the reads' addresses and seldom
+    // meet in one bank.
+    const int sink = TRI + sub;
+    walk_increments_mfma<P, NR, RS, G>(smem_raw, group_bytes, rows_off, tri_off, lane, L, ns, inv);
     __syncthreads();
     int pos = rb;  // at(sub + t, t); at(i + 1, k + 1) = at(i, k) + i + 2
     for (int t0 = 0; t0 < Lw; t0 += 8) {
+      const int q = sub + 2 + t0;
       float2 d[8];
-      int pu[8];
+      int pc[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int t = t0 + u;
-        pu[u] = pos;
-        pos += sub + t + 2;
-        d[u] = A[min(pu[u], TRI - 1)];  // past the diagonal's end: read, never stored
+        pc[u] = min(pos, sink);
+        pos += q + u;
+        d[u] = A[pc[u]];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int t = t0 + u;
         // (the main diagonal's imaginary part, rounding noise, is never read:
         // the Cholesky takes the pivot's real part only)
-        if (sub + t < L) A[pu[u]] = make_float2(cr * inv, ci * inv);
+        A[pc[u]] = make_float2(cr, ci);
         cr += d[u].x;
         ci += d[u].y;
       }
@@ -1585,15 +1603,18 @@ static void launch_bucket(const MvArgs& M, const float2* Y, const uint32_t* mask
   const int n_pix = M.n0 * M.n1;
   // Persistent grid: as many single-wavefront workgroups per CU as can be
   // resident at once -- what the runtime reports for this kernel and LDS
-  // size, bounded by LDS (160 KB / smem) and by VGPRs (4 SIMDs x min(8, 512 /
-  // VGPRs per wave)) -- and by the tuned occupancy when one is given. The
-  // pixels are dealt to the workgroups in equal shares, so the grid is sized
-  // to fill every CU exactly: a CU with room to spare may be handed more
-  // workgroups than its neighbours (measured 704 -> 804 us at P = 48 when 12
-  // would fit but 11 per CU were launched), and one workgroup more than fits
-  // waits for another to end.
+  // size, bounded by LDS (160 KB over smem rounded up to the 1280-byte
+  // allocation granule) and by VGPRs (4 SIMDs x min(8, 512 / VGPRs per wave))
+  // -- and by the tuned occupancy when one is given. The pixels are dealt to
+  // the workgroups in equal shares, so the grid is sized to fill every CU
+  // exactly: a CU with room to spare may be handed more workgroups than its
+  // neighbours (measured 704 -> 804 us at P = 48 when 12 would fit but 11 per
+  // CU were launched), and one workgroup more than fits waits for another to
+  // end (measured 572 -> 797 us at P = 48 when 16 bytes more per workgroup,
+  // 14096 instead of 11 granules' 14080, left room for 10 but 11 were
+  // launched on the runtime's report).
   const int cap = M.occupancy > 0 ? M.occupancy : 32;
-  const int lds_cap = static_cast<int>(160 * 1024 / smem);
+  const int lds_cap = static_cast<int>(160 * 1024 / ((smem + 1279) / 1280 * 1280));
 #define USX_LAUNCH(KERNEL, ...)                                                                   \
   do {                                                                                           \
     static const int vgpr_cap = [] {                                                             \
