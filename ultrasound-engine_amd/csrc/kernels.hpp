@@ -80,6 +80,11 @@ __host__ __device__ inline bool mv_fast_config(const MvArgs& M) {
 // L = P - 1 and P in bucket b (b < 4), which sit at the back of the bucket's
 // list. counts: 16 ints; lists: (mv_max_bucket_p / 16) * n_pix ints (device scratch).
 constexpr int kFullList = 8;
+// Entries of the fast kernel's lists (grids of at most 2^21 pixels): the
+// pixel in the low kRunShift bits and, when its active channels form one
+// contiguous run, the run's first channel + 1 above them (0: not a run).
+constexpr int kRunShift = 21;
+constexpr int kPixMask = (1 << kRunShift) - 1;
 void launch_mv(const MvArgs& M, const float2* Y, const uint32_t* mask, float2* frame, float* acc,
                int* counts, int* lists, hipStream_t st);
 // Largest covariance size bucket (L rounded up to 16) a configuration can produce.

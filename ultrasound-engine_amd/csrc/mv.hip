@@ -307,6 +307,59 @@ __device__ __forceinline__ Staged stage_group(const MvArgs& M, const float2* __r
   return Staged{na, dhi - dlo + 1, -dlo};
 }
 
+// stage_group for pixels whose active channels form one run from channel lo
+// (every group of the wavefront): a single round of loads -- the pixel's mask
+// words (for n_a) and each row's entries over the RS channels from lo (the
+// run's longest, clamped to the channel count; those past the run are
+// dropped), issued together -- so no active list and no load that waits on
+// another. Rows as stage_group's.
+template <int NR, int RS, int G>
+__device__ __forceinline__ Staged stage_group_run(const MvArgs& M, const float2* __restrict__ Y,
+                                                  const uint32_t* __restrict__ mask, int pix, int lo,
+                                                  int lane, float2* rows) {
+  constexpr int RA = NR / 2;
+  constexpr int LP = 64 / G;
+  constexpr int NS = (RS + LP - 1) / LP;  // channel slots per lane
+  const int sub = lane % LP;
+  const int n_ch = M.n_ch;
+  const int nw = (n_ch + 31) / 32;  // <= 8
+  const int i0 = pix / M.n1;
+  const int j = pix - i0 * M.n1;
+  const uint32_t* mp = mask + static_cast<size_t>(pix) * nw;
+  const int dlo = max(-RA, -j), dhi = min(RA, M.n1 - 1 - j);  // in-grid depths j + d
+  const size_t line = static_cast<size_t>(i0) * M.n1 + j;
+  const uint32_t cw = sub < nw ? mp[sub] : 0u;  // lane w of the group: mask word w
+  float2 v[NS][NR];
+  uint32_t mw[NS][NR];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int ch = min(lo + sub + s * LP, n_ch - 1);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const size_t p = line + min(dlo + r, dhi);
+      v[s][r] = Y[p * n_ch + ch];
+      mw[s][r] = mask[p * nw + (ch >> 5)];
+    }
+  }
+  const int na = static_cast<int>(group_sum<G>(static_cast<float>(__popc(cw))));  // exact: <= 256
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int a = sub + s * LP, ch = lo + a;
+    if (a < na) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int d = dlo + r;
+        // a channel inactive at the neighbouring depth contributes 0 there
+        // (beamformer.cpp:469); the centre's channels are active by definition
+        const bool on = d <= dhi && (d == 0 || ((mw[s][r] >> (ch & 31)) & 1u));
+        rows[r * RS + a] = on ? v[s][r] : make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+  __syncthreads();
+  return Staged{na, dhi - dlo + 1, -dlo};
+}
+
 }  // namespace
 
 // Bytes of dynamic LDS for one wavefront of bucket P: the full matrix with
@@ -381,9 +434,14 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   const bool valid = pix < n_pix;
   const int nw = (M.n_ch + 31) / 32;
   const uint32_t* mp = mask + static_cast<size_t>(valid ? pix : 0) * nw;
-  int na = 0;
+  int na = 0, lo = -1, hi = -1;  // active channels, the first and the last
   if (valid)
-    for (int w = 0; w < nw; ++w) na += __popc(mp[w]);
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t bits = mp[w];
+      na += __popc(bits);
+      if (bits && lo < 0) lo = w * 32 + __ffs(bits) - 1;
+      if (bits) hi = w * 32 + 31 - __clz(bits);
+    }
   if (valid && na == 0 && !M.incoherent) frame[pix] = make_float2(0.0f, 0.0f);
   if (valid && M.status) M.status[pix] = na == 0 ? 3 : (na < 4 ? 1 : 0);
   if (valid && na > 0 && na < 4 && M.weights) {
@@ -424,7 +482,11 @@ __global__ __launch_bounds__(256) void mv_classify(MvArgs M, const float2* __res
   const bool full = mv_fast_config(M) && b < 4 && L >= 16 * (b + 1) - 1;
   const int cb = full ? kFullList + b : b;
   const int slot = bucket_slot(counts, take ? cb : -1, take);
-  if (take) lists[static_cast<size_t>(b) * n_pix + (full ? n_pix - 1 - slot : slot)] = pix;
+  // the fast kernel's entries carry the start of a contiguous run
+  const bool run = mv_fast_config(M) && b < 4 && n_pix <= kPixMask + 1 && hi - lo + 1 == na;
+  if (take)
+    lists[static_cast<size_t>(b) * n_pix + (full ? n_pix - 1 - slot : slot)] =
+        run ? pix | ((lo + 1) << kRunShift) : pix;
 }
 
 // Trailing update of the general path's Cholesky block [j0, j0 + 8) on MFMA,
@@ -1145,7 +1207,7 @@ __device__ __forceinline__ void walk_increments_mfma(unsigned char* smem, size_t
 }
 
 template <int P, int NR, int RS, int G, bool REF>
-__global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv_solve_fast(MvArgs M, const float2* __restrict__ Y,
                                                     const uint32_t* __restrict__ mask,
                                                     float2* __restrict__ frame, float* acc,
                                                     const int* __restrict__ counts,
@@ -1181,14 +1243,27 @@ __global__ __launch_bounds__(64) void mv_solve_fast(MvArgs M, const float2* __re
   constexpr bool BORDER = decltype(border_tag)::value;
   const int count = counts[BORDER ? BUCKET : kFullList + BUCKET];
   const int* list = lists + static_cast<size_t>(BUCKET) * n_pix;
-  for (int it = xcd_slot(M.n_xcd) * G; it < count; it += gridDim.x * G) {
-    // a group past the end of the list repeats the wavefront's first pixel
-    // and stores nothing
+  // a group past the end of the list repeats the wavefront's first pixel and
+  // stores nothing; each round reads the next round's entry ahead
+  auto entry = [&](int it) {
+    const int q = it + g < count ? it + g : it;
+    return it < count ? list[BORDER ? q : n_pix - 1 - q] : 0;
+  };
+  const int stride = gridDim.x * G;
+  int next = entry(xcd_slot(M.n_xcd) * G);
+  for (int it = xcd_slot(M.n_xcd) * G; it < count; it += stride) {
     const bool valid = it + g < count;
-    const int q = valid ? it + g : it;
-    const int pix = list[BORDER ? q : n_pix - 1 - q];
+    const int e = G == 1 ? __builtin_amdgcn_readfirstlane(next) : next;  // one pixel: uniform
+    next = entry(it + stride);
+    const int pix = e & kPixMask, lo1 = e >> kRunShift;
     __syncthreads();  // LDS of the previous pixels is free
-    const Staged sg = stage_group<NR, RS, G>(M, Y, mask, pix, lane, act, rows);
+    Staged sg;
+    if constexpr (G == 1 && RS <= 2 * LP) {  // a run's single round of loads, registers allowing
+      if (__all(lo1 > 0)) sg = stage_group_run<NR, RS, G>(M, Y, mask, pix, lo1 - 1, lane, rows);
+      else sg = stage_group<NR, RS, G>(M, Y, mask, pix, lane, act, rows);
+    } else {
+      sg = stage_group<NR, RS, G>(M, Y, mask, pix, lane, act, rows);
+    }
     const int na = sg.na, nrow = sg.nrow;
     const int L = min(max(M.subarray, 2), max(2, na / 2));
     const int ns = na - L + 1;
