@@ -1446,6 +1446,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
     // by readlane from the registers of lane j0+c), in increasing k as the
     // reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
+    // One bordered pixel per wavefront: the rows above a column's pivot (and
+    // past the bordered matrix) may hold any value -- their sums and factor
+    // entries are never stored, broadcast or read (a broadcast reads the pivot
+    // row, the trailing update the stored entries, and the bordering rows lie
+    // below every pivot) -- so they go unmasked.
+    constexpr bool LOOSE = BORDER && G == 1;
     float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
     // !BORDER: the forward substitutions' vectors, lane i holding entry i
     float b1r = (!BORDER && sub < L) ? 1.0f : 0.0f, b1i = 0.0f;
@@ -1476,7 +1482,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
           const int jc = j0s + c;
-          const bool on = rowon && jc <= sub;
+          const bool on = LOOSE || (rowon && jc <= sub);
           const float2 v = A[min(rb + jc, TRI - 1)];
           sr[c] = on ? v.x : 0.0f;
           si[c] = on ? v.y : 0.0f;
@@ -1489,7 +1495,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
       for (int c = 0; c < CB; ++c) {
         const int j = j0s + c;
         if (j >= Lw) break;
-        const bool col = j < L;
+        const bool col = G == 1 || j < L;  // one pixel: j < Lw = L
 #pragma unroll
         for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
           const float br = group_bcast<G>(lr[q], j, lane), bi = group_bcast<G>(li[q], j, lane);
@@ -1507,7 +1513,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
         // rows not below the pivot take l = 0: their entries of this column are
         // never read (rows above j are not stored, and the factor's diagonal is
         // not read back), so b -= l y leaves them as they are without a select
-        const float ar = below ? sr[c] * rj : 0.0f, ai = below ? si[c] * rj : 0.0f;
+        // (LOOSE: no b, and any value will do)
+        const float ar = (LOOSE || below) ? sr[c] * rj : 0.0f;
+        const float ai = (LOOSE || below) ? si[c] * rj : 0.0f;
         lr[c] = ar;
         li[c] = ai;
         if constexpr (!BORDER) {
