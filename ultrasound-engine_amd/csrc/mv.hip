@@ -1446,12 +1446,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
     // by readlane from the registers of lane j0+c), in increasing k as the
     // reference (beamformer.cpp:48-63).
     constexpr int CB = 8;
-    // One bordered pixel per wavefront: the rows above a column's pivot (and
-    // past the bordered matrix) may hold any value -- their sums and factor
-    // entries are never stored, broadcast or read (a broadcast reads the pivot
-    // row, the trailing update the stored entries, and the bordering rows lie
-    // below every pivot) -- so they go unmasked.
-    constexpr bool LOOSE = BORDER && G == 1;
+    // Bordered pixels: the rows above a column's pivot (and past the bordered
+    // matrix, and a group's columns past its size) may hold any value -- their
+    // sums and factor entries are never stored, broadcast or read (a broadcast
+    // reads the pivot row, the trailing update the stored entries, and the
+    // bordering rows lie below every pivot) -- so they go unmasked.
+    constexpr bool LOOSE = BORDER;
     float rjj = 0.0f;  // 1 / L_jj of this lane's row (the refinement's solves)
     // !BORDER: the forward substitutions' vectors, lane i holding entry i
     float b1r = (!BORDER && sub < L) ? 1.0f : 0.0f, b1i = 0.0f;
