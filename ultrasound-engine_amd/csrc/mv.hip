@@ -1490,20 +1490,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
       }
       // Column by column, branch-free (selects): rows below the pivot take
       // L_ij (rows L and L + 1 included: the forward substitutions). A group
-      // whose pixel has no column j (j >= L) changes nothing. One pixel per
-      // wavefront (LB): each column is stored as soon as it is finished, and
-      // the pivot row's in-block entries k <= j - 2 come back from LDS as
-      // broadcasts -- only k = j - 1 by readlane; several
-      // pixels per wavefront take every term by ds_bpermute and store the
-      // block at its end.
-      constexpr bool LB = G == 1;
-      float2 rowv[CB][CB];  // LB: rowv[c][q] = L(j0 + c, j0 + q)
+      // whose pixel has no column j (j >= L) changes nothing. Each column is
+      // stored as soon as it is finished, and the pivot row's in-block
+      // entries k <= j - 2 come back from LDS as one broadcast per group --
+      // only k = j - 1 by group_bcast (readlane, ds_bpermute).
+      float2 rowv[CB][CB];  // rowv[c][q] = L(j0 + c, j0 + q)
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         const int j = j0s + c;
         if (j >= Lw) break;
         const bool col = G == 1 || j < L;  // one pixel: j < Lw = L
-        if constexpr (LB) {  // row j, columns j0 .. j - 2 (stored by now)
+        {  // row j, columns j0 .. j - 2 (stored by now)
           const int nb = tri_at(j, j0s);
 #pragma unroll
           for (int q = 0; q + 1 < c; ++q) rowv[c][q] = A[nb + q];
@@ -1511,7 +1508,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
 #pragma unroll
         for (int q = 0; q < c; ++q) {  // in-block terms, k = j0 + q
           float br, bi;
-          if (LB && q + 1 < c) {
+          if (q + 1 < c) {
             br = rowv[c][q].x;
             bi = rowv[c][q].y;
           } else {
@@ -1550,14 +1547,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
           b2r = piv ? y2r : n2r;
           b2i = piv ? y2i : n2i;
         }
-        if constexpr (LB) {
-          if (sub >= j && sub < L2) A[rb + j] = make_float2(ar, ai);
-        }
-      }
-      if (!LB && sub >= j0s && sub < L2) {
-#pragma unroll
-        for (int c = 0; c < CB; ++c)
-          if (j0s + c < L && j0s + c <= sub) A[rb + j0s + c] = make_float2(lr[c], li[c]);
+        if (sub >= j && sub < L2) A[rb + j] = make_float2(ar, ai);  // (j >= L: rows never read)
       }
       __syncthreads();
       if (!__any(ok)) break;  // every group's factorisation failed
