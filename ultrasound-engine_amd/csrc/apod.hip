@@ -271,7 +271,8 @@ __device__ __forceinline__ void activate(const f4 (&acc)[kMT][NT], u4 (&act)[kKB
 constexpr int kActU4 = kInferWaves * kKB * 64;  // one activation buffer (u4), 64 KB
 
 size_t apod_infer_smem_bytes() {
-  return sizeof(u4) * 2 * kActU4 + sizeof(float2) * kInferWaves * kInferWaves * 16;
+  return sizeof(u4) * 2 * kActU4 + sizeof(float2) * kInferWaves * kInferWaves * 16 +
+         sizeof(float) * kL * kF;  // + the biases
 }
 
 // Fragments (l, 2 wv + q, kb) through a buffer resource: one lane offset
@@ -335,27 +336,30 @@ __device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out,
              pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
     } else {
       // o of channels 16 wv + 2 g + {0, 1} (tile 2 wv) and 16 wv + 8 + 2 g +
-      // {0, 1} (tile 2 wv + 1) of pixel r of sample tile t + u
+      // {0, 1} (tile 2 wv + 1) of pixel r of sample tile t + u. The mask word
+      // and both channel pairs are loaded together (a pixel past the grid
+      // reads the last one's, and contributes nothing); inactive channels
+      // (whose entries may hold anything) are selected away.
       const long pix = base + 16 * (t + u) + r;
+      const long pc = min(pix, static_cast<long>(n_pix) - 1);
+      const int nw = (n_ch + 31) / 32;
+      const int c0 = 16 * wv + 2 * g;
+      const uint32_t mw = pix < n_pix ? mask[pc * nw + (c0 >> 5)] : 0u;
+      const float4 yv[2] = {*reinterpret_cast<const float4*>(Y + pc * n_ch + c0),
+                            *reinterpret_cast<const float4*>(Y + pc * n_ch + c0 + 8)};
       float lr = 0.0f, li = 0.0f;
-      if (pix < n_pix) {
-        const int nw = (n_ch + 31) / 32;
-        const int c0 = 16 * wv + 2 * g;
-        const uint32_t mw = mask[pix * nw + (c0 >> 5)];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int ch = c0 + 8 * h;
-          const uint32_t bits = mw >> (ch & 31);
-          if (!(bits & 3u)) continue;
-          float4 y = *reinterpret_cast<const float4*>(Y + pix * n_ch + ch);
-          if (!(bits & 1u)) y.x = y.y = 0.0f;
-          if (!(bits & 2u)) y.z = y.w = 0.0f;
-          const f4 o = h ? b : a;
-          const float g0r = (bits & 1u) ? 1.0f + o[0] : 0.0f, g0i = (bits & 1u) ? o[1] : 0.0f;
-          const float g1r = (bits & 2u) ? 1.0f + o[2] : 0.0f, g1i = (bits & 2u) ? o[3] : 0.0f;
-          lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
-          li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
-        }
+      for (int h = 0; h < 2; ++h) {
+        const int ch = c0 + 8 * h;
+        const uint32_t bits = mw >> (ch & 31);
+        const bool on0 = bits & 1u, on1 = bits & 2u;
+        const float4 y = {on0 ? yv[h].x : 0.0f, on0 ? yv[h].y : 0.0f, on1 ? yv[h].z : 0.0f,
+                          on1 ? yv[h].w : 0.0f};
+        const f4 o = h ? b : a;
+        const float g0r = on0 ? 1.0f + o[0] : 0.0f, g0i = on0 ? o[1] : 0.0f;
+        const float g1r = on1 ? 1.0f + o[2] : 0.0f, g1i = on1 ? o[3] : 0.0f;
+        lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
+        li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
       }
 #pragma unroll
       for (int o = 16; o < 64; o <<= 1) {
@@ -413,6 +417,10 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   u4* act0 = smem;
   u4* act1 = smem + kActU4;
   float2* part = reinterpret_cast<float2*>(smem + 2 * kActU4);
+  // the biases in LDS: a layer's bias reads then wait on no vector-memory
+  // load (in-order counters would hold them behind the weights in flight)
+  float* sbias = reinterpret_cast<float*>(part + kInferWaves * kInferWaves * 16);
+  for (int q = threadIdx.x; q < kL * kF; q += 64 * kInferWaves) sbias[q] = N.bias[q];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int g = lane >> 4, r = lane & 15;
@@ -435,13 +443,13 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   __syncthreads();
   while (tile < n_tiles) {
     const long base = static_cast<long>(tile) * (16 * kInferWaves);
-    ws_layer<false>(wa, N.bias + 0 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<false>(wa, sbias + 0 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
     load_weights(wsrc, 2, wv, lane, wa);
     __syncthreads();
-    ws_layer<false>(wb, N.bias + 1 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<false>(wb, sbias + 1 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
     load_weights(wsrc, 3, wv, lane, wb);
     __syncthreads();
-    ws_layer<false>(wa, N.bias + 2 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<false>(wa, sbias + 2 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
     __syncthreads();
     // act0 is free: fetch the next tile's channel vectors during the
     // last layer, then stage them
@@ -449,7 +457,7 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     const long npix = static_cast<long>(next) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
     fetch_pixel(Y, mask, n_ch, npix, npix < n_pix, lane, R);
-    ws_layer<true>(wb, N.bias + 3 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<true>(wb, sbias + 3 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
     PixelSums nps;
     const int nna = stage_input(R, wv, lane, act0, &nps);
     load_weights(wsrc, 0, wv, lane, wa);
