@@ -161,13 +161,15 @@ __device__ __forceinline__ float2 sample(const void* data, size_t base, int n, f
 // reference's a (1 - f) + b f (beamformer.cpp:27-39) to an ulp or so.
 template <bool IQ>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t trace_rsrc(const void* data, size_t base, int n) {
-  constexpr int kBytes = IQ ? 8 : 4;
+  constexpr unsigned kBytes = IQ ? 8 : 4;
   char* p = const_cast<char*>(static_cast<const char*>(data)) + base * kBytes;
   // dword 3 of a raw buffer descriptor on gfx9-family parts (32-bit data, no swizzle)
-  return __builtin_amdgcn_make_buffer_rsrc(p, 0, n * kBytes, 0x00020000);
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(static_cast<unsigned>(n) * kBytes),
+                                           0x00020000);
 }
 template <bool IQ, bool CHK = true>
-__device__ __forceinline__ float2 sample_rsrc(__amdgpu_buffer_rsrc_t t, float nm1, float idx) {
+__device__ __forceinline__ float2 sample_rsrc(__amdgpu_buffer_rsrc_t t, float nm1, float idx,
+                                              uint32_t soff = 0) {
   typedef unsigned int u2 __attribute__((ext_vector_type(2)));
   typedef unsigned int u4 __attribute__((ext_vector_type(4)));
   // Outside [0, n - 1) (and for NaN) the index moves to 2^24: its byte
@@ -179,11 +181,11 @@ __device__ __forceinline__ float2 sample_rsrc(__amdgpu_buffer_rsrc_t t, float nm
   const int k = static_cast<int>(id);             // == floor(id): id >= 0
   const float f = __builtin_amdgcn_fractf(id);    // id - floor(id), exact
   if (IQ) {
-    const u4 ab = __builtin_amdgcn_raw_buffer_load_b128(t, k * 8, 0, 0);
+    const u4 ab = __builtin_amdgcn_raw_buffer_load_b128(t, k * 8, soff, 0);
     const float ax = __uint_as_float(ab.x), ay = __uint_as_float(ab.y);
     return make_float2(fmaf(f, __uint_as_float(ab.z) - ax, ax), fmaf(f, __uint_as_float(ab.w) - ay, ay));
   } else {
-    const u2 ab = __builtin_amdgcn_raw_buffer_load_b64(t, k * 4, 0, 0);
+    const u2 ab = __builtin_amdgcn_raw_buffer_load_b64(t, k * 4, soff, 0);
     const float a = __uint_as_float(ab.x);
     return make_float2(fmaf(f, __uint_as_float(ab.y) - a, a), 0.0f);
   }
@@ -346,6 +348,15 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
   // the group's loads issued together (a slot past nsl reads slot 0's trace
   // and is not added; slots past NE are not formed at all)
   constexpr int G = (NE == 11 || NE == 12) ? 6 : (NE < 4 ? NE : 4);
+  // Consecutive events whose gathers all lie inside the traces (!SEL, !CHK):
+  // one buffer resource for the channel, based at its trace in event ev_first
+  // and spanning the launch's events, with slot s's trace reached by the
+  // scalar offset s (n_ch n_s) samples (the host keeps 16 events' traces
+  // within 4 GiB). With bounds tests each trace keeps its own resource, whose
+  // range test returns 0 past the trace.
+  const uint32_t ev_bytes = static_cast<uint32_t>(A.n_ch) * A.n_s * (IQ ? 8u : 4u);
+  const __amdgpu_buffer_rsrc_t chr = trace_rsrc<IQ>(
+      data, (static_cast<size_t>(ev_first) * A.n_ch + ch) * A.n_s, (NE - 1) * A.n_ch * A.n_s + A.n_s);
 #pragma unroll
   for (int s0 = 0; s0 < NE; s0 += G) {
     if (s0 < nsl) {
@@ -360,6 +371,8 @@ __device__ __forceinline__ float2 channel_value(const BfArgs& A, const void* dat
         const size_t base = (static_cast<size_t>(ev) * A.n_ch + ch) * A.n_s;
         if (SEL) {
           v[q] = sample<IQ>(data, base, A.n_s, idx);  // per-lane event: per-lane trace
+        } else if (!CHK) {
+          v[q] = sample_rsrc<IQ, CHK>(chr, nm1, idx, static_cast<uint32_t>(on ? s : 0) * ev_bytes);
         } else {
           v[q] = sample_rsrc<IQ, CHK>(trace_rsrc<IQ>(data, base, A.n_s), nm1, idx);
         }

@@ -199,6 +199,11 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->n_channels <= 0) fail(c, USX_ERR_INVALID, "Probe.n_elements must be > 0");
   if (b->n_channels > 512) fail(c, USX_ERR_INVALID, "beamform: at most 512 channels are supported");
   if (b->n_samples <= 0) fail(c, USX_ERR_INVALID, "Acquisition.n_samples must be > 0");
+  // the beamforming kernels reach the traces of up to 16 consecutive events
+  // through one buffer resource and 32-bit offsets
+  if (16ull * static_cast<unsigned long long>(b->n_channels) * b->n_samples * (b->is_iq ? 8u : 4u) >
+      0xFFFFFFFFull)
+    fail(c, USX_ERR_INVALID, "beamform: at most 256 MiB of channel data per transmit event");
   if (b->event_begin < 0 || b->event_end < b->event_begin || b->event_end > b->n_events)
     fail(c, USX_ERR_INVALID, "beamform: event range [%d, %d) outside [0, %d]", b->event_begin,
          b->event_end, b->n_events);
