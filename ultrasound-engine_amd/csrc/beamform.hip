@@ -250,8 +250,10 @@ __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, boo
 // vmcnt(0) wait per channel.)
 struct WaveEx {
   float r0, r1;
-  __device__ __forceinline__ void load(const BfArgs& A, int first, int lane) {
-    const int c0 = first + 4 * lane, c1 = first + 4 * (lane + 64);
+  // lane i: channel first + 32 (i / 8) + 8 w + i % 8 (and i + 64): the
+  // wavefront's 8-channel slice of each 32-channel block from first
+  __device__ __forceinline__ void load(const BfArgs& A, int first, int w, int lane) {
+    const int c0 = first + 32 * (lane >> 3) + 8 * w + (lane & 7), c1 = c0 + 32 * 8;
     r0 = c0 < A.n_ch ? A.ex[c0] : 0.0f;
     r1 = c1 < A.n_ch ? A.ex[c1] : 0.0f;
     // wait here (vmcnt(0)), once, so that the channel loop's readlanes of
@@ -537,7 +539,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
                                                           int n_slots, int select, int first,
                                                           int rect) {
   __shared__ TxSlot shared_slots[NE][64];
-  __shared__ float2 tile[64][33];
+  __shared__ float2 slice[4][64][9];  // each wavefront's 64 pixels x 8 channels (+1: banks)
   __shared__ uint32_t bits[64][kMaskWordsMax];
   int i0, j;
   tile_of(A, &i0, &j);
@@ -559,21 +561,25 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   // the channel loop twice: without bounds tests when every gather of the
   // wavefront is inside the traces, with them otherwise (gathers_inside is
   // wavefront-uniform and the four wavefronts share pixels and channel range,
-  // so all take the same copy and meet the same barriers)
+  // so all take the same copy). Each 32-channel block: wavefront w forms
+  // channels c0 + 8 w .. c0 + 8 w + 7 of its 64 pixels and writes them out
+  // through its own LDS slice (8 pixels x 64 contiguous bytes per store), so
+  // the wavefronts meet at no barrier until the mask words.
   WaveEx exs{0.0f, 0.0f};
-  if (lo <= hi) exs.load(A, (lo & ~31) + w, lane);  // this wave's channels (lo & ~31) + w + 4 i
+  const int cb = lo & ~31;
+  if (lo <= hi) exs.load(A, cb, w, lane);
   auto channels = [&](auto checked, auto t0_zero) {
     constexpr bool CHK = decltype(checked)::value;
     constexpr bool T0Z = decltype(t0_zero)::value;
-    for (int c0 = lo & ~31; c0 <= hi; c0 += 32) {
-      // each wave: channels c0 + w + 4k of this lane's pixel; their mask bits
-      // (bit cc of word c0 / 32) gathered in a register, one LDS OR per block
+    for (int c0 = cb; c0 <= hi; c0 += 32) {
       uint32_t mybits = 0u;
-      for (int cc = w; cc < 32; cc += 4) {
-        const int ch = c0 + cc;
+      const int ib = (c0 - cb) >> 2;  // the block's first index in exs
+#pragma unroll 1
+      for (int k = 0; k < 8; ++k) {
+        const int ch = c0 + 8 * w + k;
         float2 y = make_float2(0.0f, 0.0f);
         if (valid && ch >= lo && ch <= hi) {
-          const float ex = exs.at((ch - (lo & ~31)) >> 2);
+          const float ex = exs.at(ib + k);
           bool on;
           float a = rx_apod<HANN>(A, ap, ex, p, &on);
           if (rect) {  // beamformer.cpp:385: a > 0 ? 1 : 0, so a NaN weight is inactive here
@@ -581,24 +587,25 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
             a = 1.0f;
           }
           if (on) {
-            mybits |= 1u << cc;
+            mybits |= 1u << (8 * w + k);
             y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, ex, a, ev_first)
                     : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, ex, a, ev_first);
           }
         }
-        tile[lane][cc] = y;
+        slice[w][lane][k] = y;
       }
       if (mybits) atomicOr(&bits[lane][c0 >> 5], mybits);
-      __syncthreads();
-      // write the block: 64 pixel rows x 32 channels, 4 lanes per 32-byte piece
-      const int ncols = min(32, A.n_ch - c0);
-      for (int q = tid; q < 64 * 32; q += 256) {
-        const int r = q >> 5, cc = q & 31;
+      __builtin_amdgcn_wave_barrier();  // (one wavefront's LDS writes, then its reads)
+      // write the slice: 8 pixels x 8 channels per store, 64 contiguous bytes a pixel
+      const int ncols = min(8, A.n_ch - (c0 + 8 * w));
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int r = 8 * q + (lane >> 3), kk = lane & 7;
         int ir, jr;
         tile_pixel(A, tix, r, &ir, &jr);
-        if (cc < ncols && jr < A.g.n1 && ir < A.g.n0) {
-          float2* dst = Y + (static_cast<size_t>(ir) * A.g.n1 + jr) * A.n_ch + c0 + cc;
-          float2 v = tile[r][cc];
+        if (kk < ncols && jr < A.g.n1 && ir < A.g.n0) {
+          float2* dst = Y + (static_cast<size_t>(ir) * A.g.n1 + jr) * A.n_ch + c0 + 8 * w + kk;
+          float2 v = slice[w][r][kk];
           if (!first) {
             const float2 prev = *dst;
             v.x += prev.x;
@@ -607,7 +614,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
           *dst = v;
         }
       }
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();  // (the reads before the next block's writes)
     }
   };
   if (lo <= hi) {
