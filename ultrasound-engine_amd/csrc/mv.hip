@@ -1409,18 +1409,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void mv
       diag->x += eps;
       const float2* cen = rows + sg.crow * RS + sub;
       float pr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int s0 = 0; s0 < ns; s0 += 8) {
+      int s0 = 0;
+      for (; s0 + 8 <= ns; s0 += 8) {  // whole groups of 8 snapshots, then the rest
         float2 v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const float2 x = cen[min(s0 + u, ns - 1)];
-          v[u] = s0 + u < ns ? x : make_float2(0.0f, 0.0f);
-        }
+        for (int u = 0; u < 8; ++u) v[u] = cen[s0 + u];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           pr[u & 3] += v[u].x;
           pi[u & 3] += v[u].y;
         }
+      }
+      for (; s0 < ns; ++s0) {
+        const float2 x = cen[s0];
+        pr[0] += x.x;
+        pi[0] += x.y;
       }
       b2r = ((pr[0] + pr[1]) + (pr[2] + pr[3])) / static_cast<float>(ns);
       b2i = ((pi[0] + pi[1]) + (pi[2] + pi[3])) / static_cast<float>(ns);
