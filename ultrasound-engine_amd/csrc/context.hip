@@ -829,7 +829,9 @@ usx_status usx_process_frame(usx_ctx* c, const usx_demod_desc* d, const usx_bf_d
     const size_t nout = static_cast<size_t>(d->n_traces) * d->n_out;
     const size_t npix = static_cast<size_t>(b->n_axis0) * b->n_axis1;
     const auto w0 = std::chrono::steady_clock::now();
-    hip_check(c, hipEventRecord(c->ev[0], c->stream), "hipEventRecord");
+    // Stage events only when the caller asks for stage times: each record is
+    // a marker packet between the kernels, a few microseconds of idle GPU.
+    if (t) hip_check(c, hipEventRecord(c->ev[0], c->stream), "hipEventRecord");
     const float* d_rf = rf;
     float2* d_out = reinterpret_cast<float2*>(frame);
     if (memory != USX_DEVICE) {
@@ -842,14 +844,14 @@ usx_status usx_process_frame(usx_ctx* c, const usx_demod_desc* d, const usx_bf_d
     float2* d_iq = static_cast<float2*>(ensure(c, c->iq, nout * sizeof(float2)));
     run_demod(c, d, d_rf, d_iq);
     check_launch(c, "demodulate launch");
-    hip_check(c, hipEventRecord(c->ev[1], c->stream), "hipEventRecord");
+    if (t) hip_check(c, hipEventRecord(c->ev[1], c->stream), "hipEventRecord");
     run_beamform(c, b, d_iq, d_out);
     check_launch(c, "beamform launch");
     if (memory != USX_DEVICE)
       hip_check(c, hipMemcpyAsync(frame, d_out, npix * sizeof(float2), hipMemcpyDeviceToHost,
                                   c->stream),
                 "hipMemcpyAsync(frame)");
-    hip_check(c, hipEventRecord(c->ev[2], c->stream), "hipEventRecord");
+    if (t) hip_check(c, hipEventRecord(c->ev[2], c->stream), "hipEventRecord");
     if (memory != USX_DEVICE || t) {
       hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     }
