@@ -228,6 +228,21 @@ def test_learned_frame_matches_oracle(usx_mod, port, O, apod_case):
 
 
 @pytest.mark.gpu
+def test_learned_frame_pixel_limit(usx_mod, apod_case):
+    """The inference kernel reaches the channel vectors (1 KiB a pixel) through
+    32-bit buffer offsets: a frame of more than 2^22 - 128 pixels is refused
+    before anything runs."""
+    pcl = usx_mod.PipelineConfig()
+    pcl.demod.decimation = 4
+    pcl.beamformer = bf_config(usx_mod, "Learned", subarray=16)
+    big = usx_mod.ScanGrid.cartesian(-0.01, 0.01, 2048, 0.005, 0.03, 2048)
+    pipe = usx_mod.ImagingPipeline(big, pcl)
+    pipe.set_apodization(np.zeros(A.N_PARAMS, np.float32))
+    with pytest.raises(ValueError, match="pixels per frame"):
+        pipe.process_frame(apod_case.raw)
+
+
+@pytest.mark.gpu
 def test_learned_combiner_falls_back_to_mv(usx_mod, apod_case):
     pc = usx_mod.PipelineConfig()
     pc.demod.decimation = 4

@@ -172,6 +172,33 @@ __device__ __forceinline__ void fetch_pixel(const float2* Y, const uint32_t* mas
     }
   }
 }
+// fetch_pixel for the inference kernel through buffer resources over Y
+// (n_pix x 128 float2) and the mask words (n_pix x 4): one 16-byte mask load,
+// then lane (g, r)'s 16 channel pairs at one lane offset plus immediates. A
+// pair with no active channel, and a pixel past the frame, reads zeros
+// without touching memory (its offset fails the range check).
+__device__ __forceinline__ void fetch_pixel_rsrc(__amdgpu_buffer_rsrc_t yrs,
+                                                 __amdgpu_buffer_rsrc_t mrs, uint32_t pix,
+                                                 int lane, RawPixel& R) {
+  const int g = lane >> 4;
+  const u4 m = __builtin_amdgcn_raw_buffer_load_b128(mrs, pix * 16u, 0, 0);
+  R.mw[0] = m[0];
+  R.mw[1] = m[1];
+  R.mw[2] = m[2];
+  R.mw[3] = m[3];
+  const uint32_t vo = (pix * kApodChannels + 2u * g) * 8u;
+#pragma unroll
+  for (int kb = 0; kb < kKB; ++kb) {
+    const uint32_t w = R.mw[kb >> 1] >> (2 * g);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t on = w & (3u << (16 * (kb & 1) + 8 * h));
+      const uint32_t o = on ? vo : 0xFFFFF000u;
+      R.v[kb][h] = __builtin_bit_cast(
+          float4, __builtin_amdgcn_raw_buffer_load_b128(yrs, o + (16 * kb + 8 * h) * 8, 0, 0));
+    }
+  }
+}
 __device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[kKB],
                                            PixelSums* sums) {
   const int g = lane >> 4;
@@ -190,7 +217,10 @@ __device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[
       v[kb][h] = y;
       sr += y.x + y.z;
       si += y.y + y.w;
-      e += y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w;
+      e = __builtin_fmaf(y.x, y.x, e);
+      e = __builtin_fmaf(y.y, y.y, e);
+      e = __builtin_fmaf(y.z, y.z, e);
+      e = __builtin_fmaf(y.w, y.w, e);
       na += __popc(bits & 3u);
     }
   }
@@ -202,24 +232,27 @@ __device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[
     na += __shfl_xor(na, o, 64);
   }
   if (sums) *sums = PixelSums{sr, si, e};
+  // conj(sum y) / |sum y| and 1 / rms through the hardware reciprocal square
+  // root (an ulp or two; the features are rounded to BF16 next)
   float cr = 1.0f, ci = 0.0f, s = 0.0f;
   if (e > 0.0f && na > 0) {
-    const float mag = sqrtf(sr * sr + si * si);
-    if (mag > 0.0f) {
-      cr = sr / mag;
-      ci = -si / mag;
+    const float m2 = __builtin_fmaf(sr, sr, si * si);
+    if (m2 > 0.0f) {
+      const float im = __builtin_amdgcn_rsqf(m2);
+      cr = sr * im;
+      ci = -si * im;
     }
-    s = 1.0f / sqrtf(e / static_cast<float>(na));
+    s = __builtin_amdgcn_rsqf(e * __builtin_amdgcn_rcpf(static_cast<float>(na)));
   }
   cr *= s;
   ci *= s;
 #pragma unroll
   for (int kb = 0; kb < kKB; ++kb) {
     const float4 a = v[kb][0], b = v[kb][1];
-    x[kb] = u4{pack2(a.x * cr - a.y * ci, a.x * ci + a.y * cr),
-               pack2(a.z * cr - a.w * ci, a.z * ci + a.w * cr),
-               pack2(b.x * cr - b.y * ci, b.x * ci + b.y * cr),
-               pack2(b.z * cr - b.w * ci, b.z * ci + b.w * cr)};
+    x[kb] = u4{pack2(__builtin_fmaf(a.x, cr, -a.y * ci), __builtin_fmaf(a.x, ci, a.y * cr)),
+               pack2(__builtin_fmaf(a.z, cr, -a.w * ci), __builtin_fmaf(a.z, ci, a.w * cr)),
+               pack2(__builtin_fmaf(b.x, cr, -b.y * ci), __builtin_fmaf(b.x, ci, b.y * cr)),
+               pack2(__builtin_fmaf(b.z, cr, -b.w * ci), __builtin_fmaf(b.z, ci, b.w * cr))};
   }
   return na;
 }
@@ -321,10 +354,18 @@ __device__ __forceinline__ void ws_products(const u4 (&w)[2][kKB], const float4&
   }
 }
 
+// The last layer's view of the tile's channel vectors: buffer resources over
+// Y and the mask words, and lane (g, r)'s byte offsets for pixel r of sample
+// tile 0 (channel pair 16 wv + 2 g, mask word wv / 2); sample tile t adds
+// 16 t pixels as a scalar offset.
+struct LastIn {
+  __amdgpu_buffer_rsrc_t yrs, mrs;
+  uint32_t yo, mo;
+};
+
 template <bool LAST>
 __device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out, int wv, int lane,
-                                          const float2* Y, const uint32_t* mask, int n_ch,
-                                          long base, int n_pix, float2* part) {
+                                          const LastIn& in, float2* part) {
   const int g = lane >> 4, r = lane & 15;
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
@@ -337,29 +378,34 @@ __device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out,
     } else {
       // o of channels 16 wv + 2 g + {0, 1} (tile 2 wv) and 16 wv + 8 + 2 g +
       // {0, 1} (tile 2 wv + 1) of pixel r of sample tile t + u. The mask word
-      // and both channel pairs are loaded together (a pixel past the grid
-      // reads the last one's, and contributes nothing); inactive channels
-      // (whose entries may hold anything) are selected away.
-      const long pix = base + 16 * (t + u) + r;
-      const long pc = min(pix, static_cast<long>(n_pix) - 1);
-      const int nw = (n_ch + 31) / 32;
-      const int c0 = 16 * wv + 2 * g;
-      const uint32_t mw = pix < n_pix ? mask[pc * nw + (c0 >> 5)] : 0u;
-      const float4 yv[2] = {*reinterpret_cast<const float4*>(Y + pc * n_ch + c0),
-                            *reinterpret_cast<const float4*>(Y + pc * n_ch + c0 + 8)};
+      // and both channel pairs are loaded together (a pixel past the frame
+      // reads zeros: range check); inactive channels (whose entries may hold
+      // anything) are selected away, so their (1 + o) multiplies a zero.
+      const uint32_t st = static_cast<uint32_t>(16 * (t + u));
+      const uint32_t mw = __builtin_amdgcn_raw_buffer_load_b32(in.mrs, in.mo, st * 16u, 0);
+      const float4 yv[2] = {
+          __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                         in.yrs, in.yo, st * (kApodChannels * 8u), 0)),
+          __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                         in.yrs, in.yo + 64, st * (kApodChannels * 8u), 0))};
+      const uint32_t wb = mw >> (2 * g);
       float lr = 0.0f, li = 0.0f;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int ch = c0 + 8 * h;
-        const uint32_t bits = mw >> (ch & 31);
+        const uint32_t bits = wb >> (16 * (wv & 1) + 8 * h);
         const bool on0 = bits & 1u, on1 = bits & 2u;
         const float4 y = {on0 ? yv[h].x : 0.0f, on0 ? yv[h].y : 0.0f, on1 ? yv[h].z : 0.0f,
                           on1 ? yv[h].w : 0.0f};
         const f4 o = h ? b : a;
-        const float g0r = on0 ? 1.0f + o[0] : 0.0f, g0i = on0 ? o[1] : 0.0f;
-        const float g1r = on1 ? 1.0f + o[2] : 0.0f, g1i = on1 ? o[3] : 0.0f;
-        lr += g0r * y.x - g0i * y.y + g1r * y.z - g1i * y.w;
-        li += g0r * y.y + g0i * y.x + g1r * y.w + g1i * y.z;
+        const float g0r = 1.0f + o[0], g1r = 1.0f + o[2];
+        lr = __builtin_fmaf(g0r, y.x, lr);
+        lr = __builtin_fmaf(-o[1], y.y, lr);
+        lr = __builtin_fmaf(g1r, y.z, lr);
+        lr = __builtin_fmaf(-o[3], y.w, lr);
+        li = __builtin_fmaf(g0r, y.y, li);
+        li = __builtin_fmaf(o[1], y.x, li);
+        li = __builtin_fmaf(g1r, y.w, li);
+        li = __builtin_fmaf(o[3], y.z, li);
       }
 #pragma unroll
       for (int o = 16; o < 64; o <<= 1) {
@@ -373,9 +419,8 @@ __device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out,
 
 template <bool LAST>
 __device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bias, const u4* in,
-                                         u4* out, int wv, int lane,
-                                         const float2* Y, const uint32_t* mask, int n_ch, long base,
-                                         int n_pix, float2* part) {
+                                         u4* out, int wv, int lane, const LastIn& li,
+                                         float2* part) {
   const int g = lane >> 4;
   const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
   const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
@@ -385,9 +430,9 @@ __device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bia
 #pragma unroll
   for (int t = 2; t < kInferWaves; t += 4) {
     ws_products(w, b0, b1, in, t, lane, acc1);
-    ws_finish<LAST>(acc0, t - 2, out, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_finish<LAST>(acc0, t - 2, out, wv, lane, li, part);
     if (t + 2 < kInferWaves) ws_products(w, b0, b1, in, t + 2, lane, acc0);
-    ws_finish<LAST>(acc1, t, out, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_finish<LAST>(acc1, t, out, wv, lane, li, part);
   }
 }
 
@@ -425,39 +470,51 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int g = lane >> 4, r = lane & 15;
   const int n_tiles = (n_pix + 16 * kInferWaves - 1) / (16 * kInferWaves);
+  (void)n_ch;  // kApodChannels: the host admits no other channel count
   // the forward fragments, 4 layers x 128 KB, as a raw buffer
   const __amdgpu_buffer_rsrc_t wsrc =
       __builtin_amdgcn_make_buffer_rsrc(N.wf, 0, kL * kLayerBf16 * 2, 0x00020000);
+  // Y and the mask words as raw buffers (the host keeps n_pix x 1 KB of
+  // channel vectors under 4 GiB): 32-bit offsets, zeros past the frame
+  LastIn lin;
+  lin.yrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(Y), 0,
+                                              static_cast<int>(static_cast<uint32_t>(n_pix) * (kApodChannels * 8u)),
+                                              0x00020000);
+  lin.mrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(mask), 0,
+                                              static_cast<int>(static_cast<uint32_t>(n_pix) * 16u),
+                                              0x00020000);
   int tile = blockIdx.x;
   u4 wa[2][kKB], wb[2][kKB];
   load_weights(wsrc, 0, wv, lane, wa);
   int na;
   PixelSums ps;
   {
-    const long pix = static_cast<long>(tile) * (16 * kInferWaves) + 16 * wv + r;
+    const uint32_t pix = static_cast<uint32_t>(tile) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
-    fetch_pixel(Y, mask, n_ch, pix, pix < n_pix, lane, R);
+    fetch_pixel_rsrc(lin.yrs, lin.mrs, pix, lane, R);
     na = stage_input(R, wv, lane, act0, &ps);
   }
   load_weights(wsrc, 1, wv, lane, wb);
   __syncthreads();
   while (tile < n_tiles) {
-    const long base = static_cast<long>(tile) * (16 * kInferWaves);
-    ws_layer<false>(wa, sbias + 0 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    const uint32_t base = static_cast<uint32_t>(tile) * (16 * kInferWaves);
+    lin.yo = ((base + r) * kApodChannels + 16 * wv + 2 * g) * 8u;
+    lin.mo = ((base + r) * 4u + (wv >> 1)) * 4u;
+    ws_layer<false>(wa, sbias + 0 * kF, act0, act1, wv, lane, lin, part);
     load_weights(wsrc, 2, wv, lane, wa);
     __syncthreads();
-    ws_layer<false>(wb, sbias + 1 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<false>(wb, sbias + 1 * kF, act1, act0, wv, lane, lin, part);
     load_weights(wsrc, 3, wv, lane, wb);
     __syncthreads();
-    ws_layer<false>(wa, sbias + 2 * kF, act0, act1, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    ws_layer<false>(wa, sbias + 2 * kF, act0, act1, wv, lane, lin, part);
     __syncthreads();
     // act0 is free: fetch the next tile's channel vectors during the
     // last layer, then stage them
     const int next = tile + gridDim.x;
-    const long npix = static_cast<long>(next) * (16 * kInferWaves) + 16 * wv + r;
+    const uint32_t npix = static_cast<uint32_t>(next) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
-    fetch_pixel(Y, mask, n_ch, npix, npix < n_pix, lane, R);
-    ws_layer<true>(wb, sbias + 3 * kF, act1, act0, wv, lane, Y, mask, n_ch, base, n_pix, part);
+    fetch_pixel_rsrc(lin.yrs, lin.mrs, npix, lane, R);
+    ws_layer<true>(wb, sbias + 3 * kF, act1, act0, wv, lane, lin, part);
     PixelSums nps;
     const int nna = stage_input(R, wv, lane, act0, &nps);
     load_weights(wsrc, 0, wv, lane, wa);
@@ -466,7 +523,7 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     // out = sum over active channels of (1 + o_c) y_c (the wavefronts' shares
     // in order); fewer than 4 active channels -> the DAS sum, as minimum
     // variance does.
-    const long pix = base + 16 * wv + r;
+    const long pix = static_cast<long>(base) + 16 * wv + r;
     if (pix < n_pix && g == 0) {
       float lr = 0.0f, li = 0.0f;
 #pragma unroll

@@ -228,6 +228,11 @@ void check_bf(usx_ctx* c, const usx_bf_desc* b) {
   if (b->combiner == 3 && b->n_channels != USX_APOD_CHANNELS)
     fail(c, USX_ERR_INVALID, "learned apodization: the network takes %d channels, got %d",
          USX_APOD_CHANNELS, b->n_channels);
+  // the inference kernel addresses the channel vectors (1 KiB a pixel) with
+  // 32-bit offsets, its last 128-pixel tile included
+  if (b->combiner == 3 && static_cast<long long>(b->n_axis0) * b->n_axis1 > usxd::kApodMaxPixels)
+    fail(c, USX_ERR_INVALID, "learned apodization: at most %lld pixels per frame, got %lld",
+         usxd::kApodMaxPixels, static_cast<long long>(b->n_axis0) * b->n_axis1);
   if (b->compounding < 0 || b->compounding > 2) fail(c, USX_ERR_INVALID, "unknown compounding");
   if (b->is_iq && !(b->demodulation_frequency > 0))
     fail(c, USX_ERR_INVALID, "IQ ChannelData needs a positive demodulation_frequency");

@@ -99,6 +99,9 @@ constexpr int kApodFeatures = 256;
 constexpr int kApodLayers = 4;
 constexpr int kApodParams = kApodLayers * kApodFeatures * kApodFeatures + kApodLayers * kApodFeatures;
 constexpr int kApodChannels = kApodFeatures / 2;
+// Largest frame for inference: its channel vectors (1 KiB a pixel, up to the
+// last 128-pixel tile) stay below 4 GiB of 32-bit buffer offsets.
+constexpr long long kApodMaxPixels = (1ll << 22) - 128;
 
 struct ApodNet {
   float* master;   // kApodParams: W_0..W_3 ([in][out]), then b_0..b_3
