@@ -199,29 +199,24 @@ __device__ __forceinline__ void fetch_pixel_rsrc(__amdgpu_buffer_rsrc_t yrs,
     }
   }
 }
-__device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[kKB],
-                                           PixelSums* sums) {
-  const int g = lane >> 4;
-  float4 v[kKB][2];
+__device__ __forceinline__ int features_of(const RawPixel& R, u4 (&x)[kKB], PixelSums* sums) {
+  // A fetched pair holds an active channel, so its 32-channel block was
+  // written by the channel-vector kernel, which stores exactly 0 for the
+  // block's inactive channels: the entries need no masking here.
+  const float4 (&v)[kKB][2] = R.v;
   float sr = 0.0f, si = 0.0f, e = 0.0f;
-  int na = 0;
+  const int na = __popc(R.mw[0]) + __popc(R.mw[1]) + __popc(R.mw[2]) + __popc(R.mw[3]);
 #pragma unroll
   for (int kb = 0; kb < kKB; ++kb) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int ch = 16 * kb + 8 * h + 2 * g;
-      const uint32_t bits = R.mw[ch >> 5] >> (ch & 31);
-      float4 y = R.v[kb][h];
-      if (!(bits & 1u)) y.x = y.y = 0.0f;
-      if (!(bits & 2u)) y.z = y.w = 0.0f;
-      v[kb][h] = y;
+      const float4 y = v[kb][h];
       sr += y.x + y.z;
       si += y.y + y.w;
       e = __builtin_fmaf(y.x, y.x, e);
       e = __builtin_fmaf(y.y, y.y, e);
       e = __builtin_fmaf(y.z, y.z, e);
       e = __builtin_fmaf(y.w, y.w, e);
-      na += __popc(bits & 3u);
     }
   }
 #pragma unroll
@@ -229,7 +224,6 @@ __device__ __forceinline__ int features_of(const RawPixel& R, int lane, u4 (&x)[
     sr += __shfl_xor(sr, o, 64);
     si += __shfl_xor(si, o, 64);
     e += __shfl_xor(e, o, 64);
-    na += __shfl_xor(na, o, 64);
   }
   if (sums) *sums = PixelSums{sr, si, e};
   // conj(sum y) / |sum y| and 1 / rms through the hardware reciprocal square
@@ -261,7 +255,7 @@ __device__ __forceinline__ int load_features(const float2* Y, const uint32_t* ma
                                              PixelSums* sums = nullptr) {
   RawPixel R;
   fetch_pixel(Y, mask, n_ch, pix, valid, lane, R);
-  return features_of(R, lane, x, sums);
+  return features_of(R, x, sums);
 }
 
 // The next layer's B operand from output tiles 2 kb and 2 kb + 1, through GELU.
@@ -355,12 +349,13 @@ __device__ __forceinline__ void ws_products(const u4 (&w)[2][kKB], const float4&
 }
 
 // The last layer's view of the tile's channel vectors: buffer resources over
-// Y and the mask words, and lane (g, r)'s byte offsets for pixel r of sample
-// tile 0 (channel pair 16 wv + 2 g, mask word wv / 2); sample tile t adds
-// 16 t pixels as a scalar offset.
+// Y and the mask words, lane (g, r)'s byte offsets for pixel r of sample
+// tile 0 (channel pair 16 wv + 2 g, mask word wv / 2; sample tile t adds 16 t
+// pixels as a scalar offset) and the position of the pair's bits in that
+// word (2 g + 16 (wv % 2); the second pair's 8 above).
 struct LastIn {
   __amdgpu_buffer_rsrc_t yrs, mrs;
-  uint32_t yo, mo;
+  uint32_t yo, mo, shift;
 };
 
 template <bool LAST>
@@ -381,19 +376,20 @@ __device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out,
       // and both channel pairs are loaded together (a pixel past the frame
       // reads zeros: range check); inactive channels (whose entries may hold
       // anything) are selected away, so their (1 + o) multiplies a zero.
+      // The selects are per channel: a select of a whole pair lets the
+      // compiler turn it into a load behind the mask word's.
       const uint32_t st = static_cast<uint32_t>(16 * (t + u));
-      const uint32_t mw = __builtin_amdgcn_raw_buffer_load_b32(in.mrs, in.mo, st * 16u, 0);
+      const uint32_t mw =
+          __builtin_amdgcn_raw_buffer_load_b32(in.mrs, in.mo, st * 16u, 0) >> in.shift;
       const float4 yv[2] = {
           __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
                                          in.yrs, in.yo, st * (kApodChannels * 8u), 0)),
           __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
                                          in.yrs, in.yo + 64, st * (kApodChannels * 8u), 0))};
-      const uint32_t wb = mw >> (2 * g);
       float lr = 0.0f, li = 0.0f;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const uint32_t bits = wb >> (16 * (wv & 1) + 8 * h);
-        const bool on0 = bits & 1u, on1 = bits & 2u;
+        const bool on0 = mw & (1u << (8 * h)), on1 = mw & (2u << (8 * h));
         const float4 y = {on0 ? yv[h].x : 0.0f, on0 ? yv[h].y : 0.0f, on1 ? yv[h].z : 0.0f,
                           on1 ? yv[h].w : 0.0f};
         const f4 o = h ? b : a;
@@ -442,7 +438,7 @@ __device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bia
 __device__ __forceinline__ int stage_input(const RawPixel& R, int wv, int lane, u4* act,
                                            PixelSums* ps) {
   u4 x[kKB];
-  const int na = features_of(R, lane, x, ps);
+  const int na = features_of(R, x, ps);
 #pragma unroll
   for (int kb = 0; kb < kKB; ++kb) act[(wv * kKB + kb) * 64 + lane] = x[kb];
   return na;
@@ -483,6 +479,7 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   lin.mrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(mask), 0,
                                               static_cast<int>(static_cast<uint32_t>(n_pix) * 16u),
                                               0x00020000);
+  lin.shift = 2 * g + 16 * (wv & 1);
   int tile = blockIdx.x;
   u4 wa[2][kKB], wb[2][kKB];
   load_weights(wsrc, 0, wv, lane, wa);

@@ -522,8 +522,11 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_fused_kernel(BfArgs 
 // Channel-vector kernel: Y[pixel][ch] over event chunks, plus the active
 // mask (one bit per channel, a > 0). rect: MV's rectangular aperture
 // (beamformer.cpp:385). Only channels inside the tile's aperture union are
-// written (the mask marks which of them are active; consumers read no
-// other entry). Y is accumulated across chunk launches; the first writes.
+// written, whole 32-channel blocks of it, an inactive channel of a written
+// block as exactly 0 (the mask marks the active ones; consumers read no
+// entry of an unwritten block, and the learned-apodization kernels sum the
+// channel pairs of written blocks unmasked). Y is accumulated across chunk
+// launches; the first writes.
 // Values are staged per 32-channel block in LDS so each pixel's row segment
 // is written as one contiguous 256-byte run. Up to 12 slots, launch bounds
 // ask for 4 workgroups (16 wavefronts) per CU, i.e. at most 128 VGPRs: the
