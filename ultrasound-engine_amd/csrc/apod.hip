@@ -315,13 +315,12 @@ __device__ __forceinline__ void load_weights(__amdgpu_buffer_rsrc_t wrs, int l, 
       w[q][kb] = __builtin_amdgcn_raw_buffer_load_b128(wrs, 16 * lane, base + (q * kKB + kb) * 1024, 0);
 }
 
-// One layer for the wavefront's two output tiles over the workgroup's
-// kInferWaves sample tiles, two sample tiles at a time (four accumulator
-// chains), software-pipelined: the MFMAs of a pair of sample tiles are
-// issued before the GELU (VALU, two quarter-rate transcendentals per value)
-// of the previous pair, so the matrix pipe and the VALU work together. LAST:
-// the wavefront's shares of sum (1 + o_c) y_c go to `part`, else the GELU'd
-// B fragments to `out`.
+// A layer runs over the workgroup's kInferWaves sample tiles two at a time
+// (four accumulator chains), software-pipelined: the MFMAs of a pair of
+// sample tiles (ws_products, here) are issued before the VALU work on the
+// previous pair's outputs (GELU, two quarter-rate transcendentals per value,
+// or the last layer's weighted sums), so the matrix pipe and the VALU work
+// together.
 __device__ __forceinline__ void ws_products(const u4 (&w)[2][kKB], const float4& b0,
                                             const float4& b1, const u4* in, int t, int lane,
                                             f4 (&acc)[2][2]) {
@@ -348,75 +347,24 @@ __device__ __forceinline__ void ws_products(const u4 (&w)[2][kKB], const float4&
   }
 }
 
-// The last layer's view of the tile's channel vectors: buffer resources over
-// Y and the mask words, lane (g, r)'s byte offsets for pixel r of sample
-// tile 0 (channel pair 16 wv + 2 g, mask word wv / 2; sample tile t adds 16 t
-// pixels as a scalar offset) and the position of the pair's bits in that
-// word (2 g + 16 (wv % 2); the second pair's 8 above).
-struct LastIn {
-  __amdgpu_buffer_rsrc_t yrs, mrs;
-  uint32_t yo, mo, shift;
-};
 
-template <bool LAST>
-__device__ __forceinline__ void ws_finish(const f4 (&acc)[2][2], int t, u4* out, int wv, int lane,
-                                          const LastIn& in, float2* part) {
-  const int g = lane >> 4, r = lane & 15;
+// K block wv of the next layer for sample tiles t, t + 1: GELU of the
+// wavefront's two output tiles.
+__device__ __forceinline__ void ws_activate(const f4 (&acc)[2][2], int t, u4* out, int wv,
+                                            int lane) {
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
     const f4 a = acc[0][u], b = acc[1][u];
-    if constexpr (!LAST) {
-      // K block wv of the next layer for sample tile t + u
-      out[((t + u) * kKB + wv) * 64 + lane] =
-          u4{pack2(gelu(a[0]), gelu(a[1])), pack2(gelu(a[2]), gelu(a[3])),
-             pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
-    } else {
-      // o of channels 16 wv + 2 g + {0, 1} (tile 2 wv) and 16 wv + 8 + 2 g +
-      // {0, 1} (tile 2 wv + 1) of pixel r of sample tile t + u. The mask word
-      // and both channel pairs are loaded together (a pixel past the frame
-      // reads zeros: range check); inactive channels (whose entries may hold
-      // anything) are selected away, so their (1 + o) multiplies a zero.
-      // The selects are per channel: a select of a whole pair lets the
-      // compiler turn it into a load behind the mask word's.
-      const uint32_t st = static_cast<uint32_t>(16 * (t + u));
-      const uint32_t mw =
-          __builtin_amdgcn_raw_buffer_load_b32(in.mrs, in.mo, st * 16u, 0) >> in.shift;
-      const float4 yv[2] = {
-          __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
-                                         in.yrs, in.yo, st * (kApodChannels * 8u), 0)),
-          __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
-                                         in.yrs, in.yo + 64, st * (kApodChannels * 8u), 0))};
-      float lr = 0.0f, li = 0.0f;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const bool on0 = mw & (1u << (8 * h)), on1 = mw & (2u << (8 * h));
-        const float4 y = {on0 ? yv[h].x : 0.0f, on0 ? yv[h].y : 0.0f, on1 ? yv[h].z : 0.0f,
-                          on1 ? yv[h].w : 0.0f};
-        const f4 o = h ? b : a;
-        const float g0r = 1.0f + o[0], g1r = 1.0f + o[2];
-        lr = __builtin_fmaf(g0r, y.x, lr);
-        lr = __builtin_fmaf(-o[1], y.y, lr);
-        lr = __builtin_fmaf(g1r, y.z, lr);
-        lr = __builtin_fmaf(-o[3], y.w, lr);
-        li = __builtin_fmaf(g0r, y.y, li);
-        li = __builtin_fmaf(o[1], y.x, li);
-        li = __builtin_fmaf(g1r, y.w, li);
-        li = __builtin_fmaf(o[3], y.z, li);
-      }
-#pragma unroll
-      for (int o = 16; o < 64; o <<= 1) {
-        lr += __shfl_xor(lr, o, 64);
-        li += __shfl_xor(li, o, 64);
-      }
-      if (g == 0) part[(wv * kInferWaves + t + u) * 16 + r] = make_float2(lr, li);
-    }
+    out[((t + u) * kKB + wv) * 64 + lane] =
+        u4{pack2(gelu(a[0]), gelu(a[1])), pack2(gelu(a[2]), gelu(a[3])),
+           pack2(gelu(b[0]), gelu(b[1])), pack2(gelu(b[2]), gelu(b[3]))};
   }
 }
 
-template <bool LAST>
+// One hidden layer: the products of a pair of sample tiles are issued
+// before the GELU of the previous pair.
 __device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bias, const u4* in,
-                                         u4* out, int wv, int lane, const LastIn& li,
-                                         float2* part) {
+                                         u4* out, int wv, int lane) {
   const int g = lane >> 4;
   const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
   const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
@@ -426,9 +374,100 @@ __device__ __forceinline__ void ws_layer(const u4 (&w)[2][kKB], const float* bia
 #pragma unroll
   for (int t = 2; t < kInferWaves; t += 4) {
     ws_products(w, b0, b1, in, t, lane, acc1);
-    ws_finish<LAST>(acc0, t - 2, out, wv, lane, li, part);
+    ws_activate(acc0, t - 2, out, wv, lane);
     if (t + 2 < kInferWaves) ws_products(w, b0, b1, in, t + 2, lane, acc0);
-    ws_finish<LAST>(acc1, t, out, wv, lane, li, part);
+    ws_activate(acc1, t, out, wv, lane);
+  }
+}
+
+// The last layer's view of the tile's channel vectors: buffer resources over
+// Y and the mask words, lane (g, r)'s byte offsets for pixel r of sample
+// tile 0 (channel pair 16 wv + 2 g, mask word wv / 2; sample tile t adds 16 t
+// pixels as a scalar offset) and the position of the pair's bits in that
+// word (2 g + 16 (wv % 2); the second pair's 8 above).
+struct LastIn {
+  __amdgpu_buffer_rsrc_t yrs, mrs;
+  uint32_t yo, mo, shift;
+};
+// What the last layer reads for sample tiles t, t + 1: lane (g, r)'s mask word
+// and its two channel pairs of pixel r (a pixel past the frame reads zeros:
+// range check).
+struct LastLoads {
+  uint32_t mw[2];
+  float4 y[2][2];
+};
+__device__ __forceinline__ void last_loads(const LastIn& in, int t, LastLoads& L) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint32_t st = static_cast<uint32_t>(16 * (t + u));
+    L.mw[u] = __builtin_amdgcn_raw_buffer_load_b32(in.mrs, in.mo, st * 16u, 0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      L.y[u][h] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                 in.yrs, in.yo + 64 * h, st * (kApodChannels * 8u), 0));
+  }
+}
+// o of channels 16 wv + 2 g + {0, 1} (tile 2 wv) and 16 wv + 8 + 2 g + {0, 1}
+// (tile 2 wv + 1) of pixel r of sample tiles t, t + 1: the wavefront's
+// shares of sum (1 + o_c) y_c go to `part`. Inactive channels (whose entries
+// may hold anything) are selected away, so their (1 + o) multiplies a zero.
+// The selects are per channel: a select of a whole pair lets the compiler
+// turn it into a load behind the mask word's.
+__device__ __forceinline__ void last_finish(const f4 (&acc)[2][2], int t, int wv, int lane,
+                                            const LastLoads& L, uint32_t shift, float2* part) {
+  const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint32_t mw = L.mw[u] >> shift;
+    float lr = 0.0f, li = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool on0 = mw & (1u << (8 * h)), on1 = mw & (2u << (8 * h));
+      const float4 yv = L.y[u][h];
+      const float4 y = {on0 ? yv.x : 0.0f, on0 ? yv.y : 0.0f, on1 ? yv.z : 0.0f, on1 ? yv.w : 0.0f};
+      const f4 o = acc[h][u];
+      const float g0r = 1.0f + o[0], g1r = 1.0f + o[2];
+      lr = __builtin_fmaf(g0r, y.x, lr);
+      lr = __builtin_fmaf(-o[1], y.y, lr);
+      lr = __builtin_fmaf(g1r, y.z, lr);
+      lr = __builtin_fmaf(-o[3], y.w, lr);
+      li = __builtin_fmaf(g0r, y.y, li);
+      li = __builtin_fmaf(o[1], y.x, li);
+      li = __builtin_fmaf(g1r, y.w, li);
+      li = __builtin_fmaf(o[3], y.z, li);
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      lr += __shfl_xor(lr, o, 64);
+      li += __shfl_xor(li, o, 64);
+    }
+    if (g == 0) part[(wv * kInferWaves + t + u) * 16 + r] = make_float2(lr, li);
+  }
+}
+
+// The last layer: as ws_layer, with each pair's channel vectors loaded one
+// pair ahead of their use, so their memory latency hides behind a pair's
+// products and the previous pair's sums.
+__device__ __forceinline__ void ws_last_layer(const u4 (&w)[2][kKB], const float* bias,
+                                              const u4* in, int wv, int lane, const LastIn& li,
+                                              float2* part) {
+  const int g = lane >> 4;
+  const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
+  const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
+  f4 acc0[2][2], acc1[2][2];
+  LastLoads L0, L1;
+  last_loads(li, 0, L0);
+  ws_products(w, b0, b1, in, 0, lane, acc0);
+#pragma unroll
+  for (int t = 2; t < kInferWaves; t += 4) {
+    last_loads(li, t, L1);
+    ws_products(w, b0, b1, in, t, lane, acc1);
+    last_finish(acc0, t - 2, wv, lane, L0, li.shift, part);
+    if (t + 2 < kInferWaves) {
+      last_loads(li, t + 2, L0);
+      ws_products(w, b0, b1, in, t + 2, lane, acc0);
+    }
+    last_finish(acc1, t, wv, lane, L1, li.shift, part);
   }
 }
 
@@ -497,21 +536,20 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     const uint32_t base = static_cast<uint32_t>(tile) * (16 * kInferWaves);
     lin.yo = ((base + r) * kApodChannels + 16 * wv + 2 * g) * 8u;
     lin.mo = ((base + r) * 4u + (wv >> 1)) * 4u;
-    ws_layer<false>(wa, sbias + 0 * kF, act0, act1, wv, lane, lin, part);
+    ws_layer(wa, sbias + 0 * kF, act0, act1, wv, lane);
     load_weights(wsrc, 2, wv, lane, wa);
     __syncthreads();
-    ws_layer<false>(wb, sbias + 1 * kF, act1, act0, wv, lane, lin, part);
+    ws_layer(wb, sbias + 1 * kF, act1, act0, wv, lane);
     load_weights(wsrc, 3, wv, lane, wb);
     __syncthreads();
-    ws_layer<false>(wa, sbias + 2 * kF, act0, act1, wv, lane, lin, part);
+    ws_layer(wa, sbias + 2 * kF, act0, act1, wv, lane);
     __syncthreads();
-    // act0 is free: fetch the next tile's channel vectors during the
-    // last layer, then stage them
+    ws_last_layer(wb, sbias + 3 * kF, act1, wv, lane, lin, part);
+    // act0 is free: fetch and stage the next tile's channel vectors
     const int next = tile + gridDim.x;
     const uint32_t npix = static_cast<uint32_t>(next) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
     fetch_pixel_rsrc(lin.yrs, lin.mrs, npix, lane, R);
-    ws_layer<true>(wb, sbias + 3 * kF, act1, act0, wv, lane, lin, part);
     PixelSums nps;
     const int nna = stage_input(R, wv, lane, act0, &nps);
     load_weights(wsrc, 0, wv, lane, wa);
