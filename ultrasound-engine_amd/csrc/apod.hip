@@ -177,11 +177,12 @@ __device__ __forceinline__ void fetch_pixel(const float2* Y, const uint32_t* mas
 // then lane (g, r)'s 16 channel pairs at one lane offset plus immediates. A
 // pair with no active channel, and a pixel past the frame, reads zeros
 // without touching memory (its offset fails the range check).
-__device__ __forceinline__ void fetch_pixel_rsrc(__amdgpu_buffer_rsrc_t yrs,
-                                                 __amdgpu_buffer_rsrc_t mrs, uint32_t pix,
-                                                 int lane, RawPixel& R) {
+__device__ __forceinline__ u4 mask_row(__amdgpu_buffer_rsrc_t mrs, uint32_t pix) {
+  return __builtin_amdgcn_raw_buffer_load_b128(mrs, pix * 16u, 0, 0);
+}
+__device__ __forceinline__ void fetch_pixel_rsrc(__amdgpu_buffer_rsrc_t yrs, const u4& m,
+                                                 uint32_t pix, int lane, RawPixel& R) {
   const int g = lane >> 4;
-  const u4 m = __builtin_amdgcn_raw_buffer_load_b128(mrs, pix * 16u, 0, 0);
   R.mw[0] = m[0];
   R.mw[1] = m[1];
   R.mw[2] = m[2];
@@ -447,10 +448,14 @@ __device__ __forceinline__ void last_finish(const f4 (&acc)[2][2], int t, int wv
 
 // The last layer: as ws_layer, with each pair's channel vectors loaded one
 // pair ahead of their use, so their memory latency hides behind a pair's
-// products and the previous pair's sums.
+// products and the previous pair's sums. Once the last products are issued
+// the weights' registers are free, and the next tile's channel vectors (at
+// pixel npix of the wavefront's next sample tile, mask row nm) are fetched
+// into R behind the last two pairs' sums.
 __device__ __forceinline__ void ws_last_layer(const u4 (&w)[2][kKB], const float* bias,
                                               const u4* in, int wv, int lane, const LastIn& li,
-                                              float2* part) {
+                                              float2* part, const u4& nm, uint32_t npix,
+                                              RawPixel& R) {
   const int g = lane >> 4;
   const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
   const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
@@ -466,6 +471,8 @@ __device__ __forceinline__ void ws_last_layer(const u4 (&w)[2][kKB], const float
     if (t + 2 < kInferWaves) {
       last_loads(li, t + 2, L0);
       ws_products(w, b0, b1, in, t + 2, lane, acc0);
+    } else {
+      fetch_pixel_rsrc(li.yrs, nm, npix, lane, R);
     }
     last_finish(acc1, t, wv, lane, L1, li.shift, part);
   }
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   {
     const uint32_t pix = static_cast<uint32_t>(tile) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
-    fetch_pixel_rsrc(lin.yrs, lin.mrs, pix, lane, R);
+    fetch_pixel_rsrc(lin.yrs, mask_row(lin.mrs, pix), pix, lane, R);
     na = stage_input(R, wv, lane, act0, &ps);
   }
   load_weights(wsrc, 1, wv, lane, wb);
@@ -544,12 +551,13 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     __syncthreads();
     ws_layer(wa, sbias + 2 * kF, act0, act1, wv, lane);
     __syncthreads();
-    ws_last_layer(wb, sbias + 3 * kF, act1, wv, lane, lin, part);
-    // act0 is free: fetch and stage the next tile's channel vectors
+    // the next tile's mask rows now, its channel vectors during the last
+    // layer's final sums; then act0 (free) takes them
     const int next = tile + gridDim.x;
     const uint32_t npix = static_cast<uint32_t>(next) * (16 * kInferWaves) + 16 * wv + r;
+    const u4 nm = mask_row(lin.mrs, npix);
     RawPixel R;
-    fetch_pixel_rsrc(lin.yrs, lin.mrs, npix, lane, R);
+    ws_last_layer(wb, sbias + 3 * kF, act1, wv, lane, lin, part, nm, npix, R);
     PixelSums nps;
     const int nna = stage_input(R, wv, lane, act0, &nps);
     load_weights(wsrc, 0, wv, lane, wa);
