@@ -250,10 +250,11 @@ __device__ __forceinline__ void aperture_range(const BfArgs& A, const Pt& p, boo
 // vmcnt(0) wait per channel.)
 struct WaveEx {
   float r0, r1;
-  // lane i: channel first + 32 (i / 8) + 8 w + i % 8 (and i + 64): the
-  // wavefront's 8-channel slice of each 32-channel block from first
+  // lane i: channel first + 32 (i / 8) + w + 4 (i % 8) (and i + 64): the
+  // wavefront's channels (every fourth, from w) of each 32-channel block from
+  // first
   __device__ __forceinline__ void load(const BfArgs& A, int first, int w, int lane) {
-    const int c0 = first + 32 * (lane >> 3) + 8 * w + (lane & 7), c1 = c0 + 32 * 8;
+    const int c0 = first + 32 * (lane >> 3) + w + 4 * (lane & 7), c1 = c0 + 32 * 8;
     r0 = c0 < A.n_ch ? A.ex[c0] : 0.0f;
     r1 = c1 < A.n_ch ? A.ex[c1] : 0.0f;
     // wait here (vmcnt(0)), once, so that the channel loop's readlanes of
@@ -542,7 +543,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
                                                           int n_slots, int select, int first,
                                                           int rect) {
   __shared__ TxSlot shared_slots[NE][64];
-  __shared__ float2 slice[4][64][9];  // each wavefront's 64 pixels x 8 channels (+1: banks)
+  __shared__ float2 blk[64][33];  // one 32-channel block of the 64 pixels (+1: banks)
   __shared__ uint32_t bits[64][kMaskWordsMax];
   int i0, j;
   tile_of(A, &i0, &j);
@@ -564,10 +565,12 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
   // the channel loop twice: without bounds tests when every gather of the
   // wavefront is inside the traces, with them otherwise (gathers_inside is
   // wavefront-uniform and the four wavefronts share pixels and channel range,
-  // so all take the same copy). Each 32-channel block: wavefront w forms
-  // channels c0 + 8 w .. c0 + 8 w + 7 of its 64 pixels and writes them out
-  // through its own LDS slice (8 pixels x 64 contiguous bytes per store), so
-  // the wavefronts meet at no barrier until the mask words.
+  // so all take the same copy, and all meet at the same barriers). Each
+  // 32-channel block: wavefront w forms channels c0 + w + 4 k (k < 8) of the
+  // 64 pixels -- every fourth channel, so a block at the aperture's edge
+  // leaves the wavefronts equal shares of its active channels -- into the
+  // block's LDS rows; after a barrier each wavefront writes 16 pixels' rows
+  // out (2 pixels x 256 contiguous bytes per store).
   WaveEx exs{0.0f, 0.0f};
   const int cb = lo & ~31;
   if (lo <= hi) exs.load(A, cb, w, lane);
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
       const int ib = (c0 - cb) >> 2;  // the block's first index in exs
 #pragma unroll 1
       for (int k = 0; k < 8; ++k) {
-        const int ch = c0 + 8 * w + k;
+        const int ch = c0 + w + 4 * k;
         float2 y = make_float2(0.0f, 0.0f);
         if (valid && ch >= lo && ch <= hi) {
           const float ex = exs.at(ib + k);
@@ -590,34 +593,39 @@ __global__ __launch_bounds__(256, NE <= 12 ? 4 : 1) void bf_channels_kernel(BfAr
             a = 1.0f;
           }
           if (on) {
-            mybits |= 1u << (8 * w + k);
+            mybits |= 1u << (w + 4 * k);
             y = sel ? channel_value<NE, IQ, true, true>(A, data, p, sl, nsl, ch, ex, a, ev_first)
                     : channel_value<NE, IQ, false, CHK, T0Z>(A, data, p, sl, nsl, ch, ex, a, ev_first);
           }
         }
-        slice[w][lane][k] = y;
+        blk[lane][w + 4 * k] = y;
       }
       if (mybits) atomicOr(&bits[lane][c0 >> 5], mybits);
-      __builtin_amdgcn_wave_barrier();  // (one wavefront's LDS writes, then its reads)
-      // write the slice: 8 pixels x 8 channels per store, 64 contiguous bytes a pixel
-      const int ncols = min(8, A.n_ch - (c0 + 8 * w));
+      __syncthreads();
+      // write the block: wavefront w's pixels 16 w .. 16 w + 15, two per store
+      // (lanes 0-31 and 32-63 the 32 channels of one pixel each), read from
+      // LDS before the barrier that lets the next block's writes in
+      const int kk = lane & 31;
+      float2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = blk[16 * w + 2 * q + (lane >> 5)][kk];
+      __syncthreads();
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int r = 8 * q + (lane >> 3), kk = lane & 7;
+        const int r = 16 * w + 2 * q + (lane >> 5);
         int ir, jr;
         tile_pixel(A, tix, r, &ir, &jr);
-        if (kk < ncols && jr < A.g.n1 && ir < A.g.n0) {
-          float2* dst = Y + (static_cast<size_t>(ir) * A.g.n1 + jr) * A.n_ch + c0 + 8 * w + kk;
-          float2 v = slice[w][r][kk];
+        if (c0 + kk < A.n_ch && jr < A.g.n1 && ir < A.g.n0) {
+          float2* dst = Y + (static_cast<size_t>(ir) * A.g.n1 + jr) * A.n_ch + c0 + kk;
+          float2 o = v[q];
           if (!first) {
             const float2 prev = *dst;
-            v.x += prev.x;
-            v.y += prev.y;
+            o.x += prev.x;
+            o.y += prev.y;
           }
-          *dst = v;
+          *dst = o;
         }
       }
-      __builtin_amdgcn_wave_barrier();  // (the reads before the next block's writes)
     }
   };
   if (lo <= hi) {
