@@ -194,8 +194,18 @@ def test_mv_weights_match_oracle_and_reproduce_the_frame(usx_mod, port, O, apod_
     assert np.array_equal(st == 0, (fb == 0) & (ex2["info"][:, 0].reshape(st.shape) >= 4))
 
 
+@pytest.fixture(scope="module")
+def apod_ragged(usx_mod, O):
+    # 23 x 61 = 1403 pixels: the inference kernel's last 128-pixel tile is
+    # short (123 pixels), so its loads past the frame must read zeros
+    return make_case(usx_mod, O, n_el=128, n_samples=1024, n_tx=3, n_scat=1500,
+                     grid=(23, 0.008, 0.030, 61), box=(-0.02, 0.02, 0.005, 0.032))
+
+
 @pytest.mark.gpu
-def test_learned_frame_matches_oracle(usx_mod, port, O, apod_case):
+@pytest.mark.parametrize("case_name", ["apod_case", "apod_ragged"])
+def test_learned_frame_matches_oracle(usx_mod, port, O, case_name, request):
+    apod_case = request.getfixturevalue(case_name)
     run_mv, pc = _runner(usx_mod, apod_case)
     d = usx_mod.DemodConfig()
     d.decimation = 4
