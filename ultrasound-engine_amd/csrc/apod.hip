@@ -300,7 +300,7 @@ constexpr int kActU4 = kInferWaves * kKB * 64;  // one activation buffer (u4), 6
 
 size_t apod_infer_smem_bytes() {
   return sizeof(u4) * 2 * kActU4 + sizeof(float2) * kInferWaves * kInferWaves * 16 +
-         sizeof(float) * kL * kF;  // + the biases
+         sizeof(float) * kL * kF + 8;  // + the biases and the tile's flag bytes
 }
 
 // Fragments (l, 2 wv + q, kb) through a buffer resource: one lane offset
@@ -446,47 +446,82 @@ __device__ __forceinline__ void last_finish(const f4 (&acc)[2][2], int t, int wv
   }
 }
 
+// A pair of sample tiles whose 32 pixels have no active channel among the
+// wavefront's 16 (bit wv of neither tile's flag byte in fl) contributes
+// nothing: its loads, products and sums are skipped and its shares are 0.
+__device__ __forceinline__ void last_zero(int t, int wv, int lane, float2* part) {
+  const int g = lane >> 4, r = lane & 15;
+  if (g == 0) {
+    part[(wv * kInferWaves + t) * 16 + r] = make_float2(0.0f, 0.0f);
+    part[(wv * kInferWaves + t + 1) * 16 + r] = make_float2(0.0f, 0.0f);
+  }
+}
+
 // The last layer: as ws_layer, with each pair's channel vectors loaded one
 // pair ahead of their use, so their memory latency hides behind a pair's
 // products and the previous pair's sums. Once the last products are issued
 // the weights' registers are free, and the next tile's channel vectors (at
 // pixel npix of the wavefront's next sample tile, mask row nm) are fetched
-// into R behind the last two pairs' sums.
+// into R behind the last two pairs' sums. fl: the tile's flag bytes (byte t:
+// the 16-channel blocks holding an active channel of sample tile t).
 __device__ __forceinline__ void ws_last_layer(const u4 (&w)[2][kKB], const float* bias,
                                               const u4* in, int wv, int lane, const LastIn& li,
                                               float2* part, const u4& nm, uint32_t npix,
-                                              RawPixel& R) {
+                                              RawPixel& R, uint64_t fl) {
   const int g = lane >> 4;
   const float4 b0 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv) + 4 * g);
   const float4 b1 = *reinterpret_cast<const float4*>(bias + 16 * (2 * wv + 1) + 4 * g);
+  auto live = [&](int t) { return (((fl >> (8 * t)) | (fl >> (8 * t + 8))) >> wv) & 1u; };
   f4 acc0[2][2], acc1[2][2];
   LastLoads L0, L1;
-  last_loads(li, 0, L0);
-  ws_products(w, b0, b1, in, 0, lane, acc0);
+  const bool on0 = live(0);
+  if (on0) {
+    last_loads(li, 0, L0);
+    ws_products(w, b0, b1, in, 0, lane, acc0);
+  }
+  bool onA = on0;
 #pragma unroll
   for (int t = 2; t < kInferWaves; t += 4) {
-    last_loads(li, t, L1);
-    ws_products(w, b0, b1, in, t, lane, acc1);
-    last_finish(acc0, t - 2, wv, lane, L0, li.shift, part);
+    const bool onB = live(t);
+    if (onB) {
+      last_loads(li, t, L1);
+      ws_products(w, b0, b1, in, t, lane, acc1);
+    }
+    if (onA) last_finish(acc0, t - 2, wv, lane, L0, li.shift, part);
+    else last_zero(t - 2, wv, lane, part);
     if (t + 2 < kInferWaves) {
-      last_loads(li, t + 2, L0);
-      ws_products(w, b0, b1, in, t + 2, lane, acc0);
+      onA = live(t + 2);
+      if (onA) {
+        last_loads(li, t + 2, L0);
+        ws_products(w, b0, b1, in, t + 2, lane, acc0);
+      }
     } else {
       fetch_pixel_rsrc(li.yrs, nm, npix, lane, R);
     }
-    last_finish(acc1, t, wv, lane, L1, li.shift, part);
+    if (onB) last_finish(acc1, t, wv, lane, L1, li.shift, part);
+    else last_zero(t, wv, lane, part);
   }
 }
 
 // Stage this wavefront's 16 pixels (sample tile wv) from their fetched
 // channel vectors into the activation buffer. Inactive channels are zero
 // features, so the first layer's products over them add exactly nothing.
+// Also this sample tile's flag byte: bit b set when one of its 16 pixels has
+// an active channel among 16 b .. 16 b + 15 (the mask words are the same in
+// the four lanes of a pixel).
 __device__ __forceinline__ int stage_input(const RawPixel& R, int wv, int lane, u4* act,
-                                           PixelSums* ps) {
+                                           PixelSums* ps, uint8_t* flags) {
   u4 x[kKB];
   const int na = features_of(R, x, ps);
 #pragma unroll
   for (int kb = 0; kb < kKB; ++kb) act[(wv * kKB + kb) * 64 + lane] = x[kb];
+  uint32_t fb = 0;
+#pragma unroll
+  for (int b = 0; b < kKB; ++b)
+    fb |= ((R.mw[b >> 1] >> (16 * (b & 1))) & 0xFFFFu) ? (1u << b) : 0u;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) fb |= __shfl_xor(fb, o, 64);
+  if (lane == 0) flags[wv] = static_cast<uint8_t>(fb);
   return na;
 }
 
@@ -507,6 +542,8 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
   // the biases in LDS: a layer's bias reads then wait on no vector-memory
   // load (in-order counters would hold them behind the weights in flight)
   float* sbias = reinterpret_cast<float*>(part + kInferWaves * kInferWaves * 16);
+  uint8_t* flags = reinterpret_cast<uint8_t*>(sbias + kL * kF);
+  static_assert(kInferWaves == 8, "one flag byte per sample tile, 8 to a 64-bit word");
   for (int q = threadIdx.x; q < kL * kF; q += 64 * kInferWaves) sbias[q] = N.bias[q];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -535,7 +572,7 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     const uint32_t pix = static_cast<uint32_t>(tile) * (16 * kInferWaves) + 16 * wv + r;
     RawPixel R;
     fetch_pixel_rsrc(lin.yrs, mask_row(lin.mrs, pix), pix, lane, R);
-    na = stage_input(R, wv, lane, act0, &ps);
+    na = stage_input(R, wv, lane, act0, &ps, flags);
   }
   load_weights(wsrc, 1, wv, lane, wb);
   __syncthreads();
@@ -543,6 +580,12 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     const uint32_t base = static_cast<uint32_t>(tile) * (16 * kInferWaves);
     lin.yo = ((base + r) * kApodChannels + 16 * wv + 2 * g) * 8u;
     lin.mo = ((base + r) * 4u + (wv >> 1)) * 4u;
+    // the tile's flag bytes, read before the first layer's barrier (the next
+    // staging, which rewrites them, follows three more barriers)
+    const uint64_t fl64 = *reinterpret_cast<const uint64_t*>(flags);
+    const uint64_t fl = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(
+                             static_cast<uint32_t>(fl64 >> 32))) << 32) |
+                        __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(fl64));
     ws_layer(wa, sbias + 0 * kF, act0, act1, wv, lane);
     load_weights(wsrc, 2, wv, lane, wa);
     __syncthreads();
@@ -557,9 +600,9 @@ __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet
     const uint32_t npix = static_cast<uint32_t>(next) * (16 * kInferWaves) + 16 * wv + r;
     const u4 nm = mask_row(lin.mrs, npix);
     RawPixel R;
-    ws_last_layer(wb, sbias + 3 * kF, act1, wv, lane, lin, part, nm, npix, R);
+    ws_last_layer(wb, sbias + 3 * kF, act1, wv, lane, lin, part, nm, npix, R, fl);
     PixelSums nps;
-    const int nna = stage_input(R, wv, lane, act0, &nps);
+    const int nna = stage_input(R, wv, lane, act0, &nps, flags);
     load_weights(wsrc, 0, wv, lane, wa);
     load_weights(wsrc, 1, wv, lane, wb);
     __syncthreads();
