@@ -526,10 +526,11 @@ __device__ __forceinline__ int stage_input(const RawPixel& R, int wv, int lane, 
 }
 
 // Persistent: workgroup b takes the 128-pixel tiles b, b + gridDim.x, ...
-// (one workgroup per CU: 136 KB of LDS). The next tile's channel vectors are
-// fetched while the last layer runs (into registers; the first layer's
-// weights, whose registers the last layer does not use, follow), so their
-// HBM latency hides behind the MFMAs.
+// (one workgroup per CU: 140 KB of LDS). The next tile's mask rows are loaded
+// before the last layer and its channel vectors once the last layer's
+// products are issued (into the registers its weights leave), so most of
+// their memory latency hides behind the last sums; the first two layers'
+// weights follow the staging.
 __global__ __launch_bounds__(64 * kInferWaves, 1) void apod_infer_kernel(ApodNet N, int n_pix, int n_ch,
                                                          const float2* __restrict__ Y,
                                                          const uint32_t* __restrict__ mask,
